@@ -1,0 +1,288 @@
+"""Benchmark: full incompressible RHS evaluations per second at 512^3 on one
+MI355X, FP32 storage with FP64-residual refinement versus FP64.
+
+One step = one full RHS evaluation (pdo_rhs_eval): 9 cd06 velocity-gradient
+passes, 9 cf90 filter passes, the cd06 divergence (3 passes), the periodic
+FFT Poisson solve with cd06 modified wavenumbers (5 passes) and the pressure
+gradient subtraction (3 passes), on a triply periodic 512^3 grid of side 2 pi.
+Input: seeded divergence-free isotropic turbulence, E(k) ~ k^4 exp(-2 (k/8)^2),
+rms 1, generated on the device; resident in HBM before timing starts.
+
+Prints ONE JSON line.  value = FP32+refine evaluations/s (whole job, 1 GPU).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8000.0  # GB/s, MI355X HBM3E
+WORDS_PER_POINT = 65  # algorithmic words per RHS evaluation
+
+
+def turbulence(n, k0, seed, dtype, device):
+    """Divergence-free random-phase field with E(k) ~ k^4 exp(-2 (k/k0)^2), rms 1."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    k1 = torch.fft.fftfreq(n, d=1.0 / n, device=device, dtype=torch.float64)
+    kr = torch.fft.rfftfreq(n, d=1.0 / n, device=device, dtype=torch.float64)
+    KX = k1.view(n, 1, 1)
+    KY = k1.view(1, n, 1)
+    KZ = kr.view(1, 1, -1)
+    K2 = KX * KX + KY * KY + KZ * KZ
+    Km = torch.sqrt(K2).clamp_min(1.0)
+    amp = torch.sqrt(Km ** 4 * torch.exp(-2.0 * (Km / k0) ** 2) / (4 * torch.pi * Km ** 2))
+    amp[0, 0, 0] = 0.0
+    uh = []
+    for _ in range(3):
+        re = torch.randn(K2.shape, generator=g, device=device, dtype=torch.float64)
+        im = torch.randn(K2.shape, generator=g, device=device, dtype=torch.float64)
+        uh.append(torch.complex(re, im) * amp)
+    kd = (KX * uh[0] + KY * uh[1] + KZ * uh[2]) / torch.where(K2 == 0, torch.ones_like(K2), K2)
+    uh = [uh[0] - KX * kd, uh[1] - KY * kd, uh[2] - KZ * kd]
+    # torch axes (a0, a1, a2) with a2 fastest are the Fortran (z, y, x): the
+    # x velocity is the component along a2
+    u = [torch.fft.irfftn(c, s=(n, n, n)) for c in uh]
+    del uh, kd
+    rms = torch.sqrt((u[0] ** 2 + u[1] ** 2 + u[2] ** 2).mean() / 3.0)
+    return [(c / rms).contiguous().view(-1).to(dtype) for c in (u[2], u[1], u[0])]
+
+
+def rel_l2(a, b):
+    import torch
+    a = a.double()
+    b = b.double()
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-breakdown", action="store_true", help="skip the per-pass timing loop")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import padeops_amd as P
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    N = n ** 3
+    L = 2 * np.pi
+    stream = torch.cuda.current_stream().cuda_stream
+
+    # inputs (FP64 master field; the FP32 path gets it rounded)
+    u64 = turbulence(n, 8.0, 1234, torch.float64, dev)
+    u32 = [c.float() for c in u64]
+
+    def buffers(dt):
+        return (torch.empty(9 * N, dtype=dt, device=dev), [torch.empty(N, dtype=dt, device=dev) for _ in range(3)])
+
+    def run(rhs, U, grad, out, steps):
+        for _ in range(steps):
+            rhs.eval(U[0], U[1], U[2], grad, *out, stream=stream)
+
+    def timed(rhs, U, grad, out):
+        run(rhs, U, grad, out, args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        run(rhs, U, grad, out, args.steps)
+        e1.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return e0.elapsed_time(e1) / args.steps, wall
+
+    # FP64 path
+    rhs64 = P.FullRHS(n, L, "f64")
+    g64, o64 = buffers(torch.float64)
+    ms64, _ = timed(rhs64, u64, g64, o64)
+    div64 = rhs64.max_divergence(*o64)
+    # FP32 + refinement path (the headline)
+    rhs32 = P.FullRHS(n, L, "f32_refine")
+    g32, o32 = buffers(torch.float32)
+    ms32, wall32 = timed(rhs32, u32, g32, o32)
+
+    # accuracy of the FP32+refine evaluation against the FP64 one
+    errs = {}
+    names = ["dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz"]
+    for q in range(9):
+        errs[names[q]] = rel_l2(g32[q * N:(q + 1) * N], g64[q * N:(q + 1) * N])
+    for name, a, b in zip(("u_proj", "v_proj", "w_proj"), o32, o64):
+        errs[name] = rel_l2(a, b)
+
+    evals64 = 1000.0 / ms64
+    evals32 = 1000.0 / ms32
+    bytes64 = WORDS_PER_POINT * N * 8
+    bytes32 = WORDS_PER_POINT * N * 4
+
+    # per-pass breakdown and the dominant kernel (HIP events on the launch stream)
+    breakdown = {}
+    roofline = None
+    if not args.no_breakdown:
+        breakdown, roofline = pass_breakdown(P, n, u32[0], stream, args.steps)
+
+    probe = None if args.no_breakdown else hbm_probe(args.steps)
+    if roofline is not None:
+        roofline["traffic"] = committed_traffic(roofline["kernel"])
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline()
+
+    line = {
+        "metric": "RHS evals/s at 512^3: FP32+refine vs FP64 (<=1e-6 rel err); HBM GB/s vs peak",
+        "value": round(evals32, 3),
+        "unit": "RHS evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 storage+solves, fp64 residual refinement",
+        "data": "synthetic: seeded divergence-free isotropic turbulence, E(k)~k^4 exp(-2(k/8)^2), rms 1",
+        "config": {"workload": "full incompressible RHS (cd06 grad-u x9, cf90 x9, cd06 div, FFT Poisson "
+                               "with cd06 modified wavenumbers, cd06 grad p), triply periodic %d^3, L=2pi" % n,
+                   "grid": n, "passes_per_eval": 29, "words_per_point": WORDS_PER_POINT},
+        "fp64_evals_per_s": round(evals64, 3),
+        "fp64_ms_per_eval": round(ms64, 4),
+        "speedup_fp32r_vs_fp64": round(evals32 / evals64, 3),
+        "rhs_achieved_GBps_fp32r": round(bytes32 / (ms32 * 1e-3) / 1e9, 1),
+        "rhs_achieved_GBps_fp64": round(bytes64 / (ms64 * 1e-3) / 1e9, 1),
+        "rhs_frac_of_peak_fp32r": round(bytes32 / (ms32 * 1e-3) / 1e9 / PEAK_HBM, 4),
+        "rhs_frac_of_peak_fp64": round(bytes64 / (ms64 * 1e-3) / 1e9 / PEAK_HBM, 4),
+        "fp32r_rel_l2_vs_fp64": {k: (None if v is None else float("%.3e" % v)) for k, v in errs.items()},
+        "fp64_max_div_after_projection": float("%.3e" % div64),
+        "roofline": roofline,
+        "pass_ms": breakdown,
+        "hbm_probe": probe,
+        "cpu_baseline": cpu,
+        "wall_s_timed_fp32r": round(wall32, 4),
+    }
+    print(json.dumps(line))
+
+
+def pass_breakdown(P, n, f32, stream, steps):
+    """Average duration of each pass type of the FP32+refine evaluation, timed
+    with HIP events on the launch stream; returns (ms per pass, roofline of
+    the dominant kernel)."""
+    import numpy as np
+    import torch
+    N = n ** 3
+    out = torch.empty_like(f32)
+    L = 2 * np.pi
+    d = P.cd06()
+    d.init(n, L / n, True)
+    c = P.cf90()
+    c.init(n, True)
+    ps = P.PoissonPeriodic()
+    ps.init(L / n, L / n, L / n, n, n, n, ("cd06",) * 3, 1)
+    calls = {
+        "cd06_x": lambda: d.dd1(f32, out, n, n, prec="f32_refine", stream=stream),
+        "cd06_y": lambda: d.dd2(f32, out, n, n, prec="f32_refine", stream=stream),
+        "cd06_z": lambda: d.dd3(f32, out, n, n, prec="f32_refine", stream=stream),
+        "cf90_x": lambda: c.filter1(f32, out, n, n, prec="f32_refine", stream=stream),
+        "cf90_y": lambda: c.filter2(f32, out, n, n, prec="f32_refine", stream=stream),
+        "cf90_z": lambda: c.filter3(f32, out, n, n, prec="f32_refine", stream=stream),
+        "poisson_5pass": lambda: ps.poisson_solve(f32, out, prec="f32_refine", stream=stream),
+    }
+    # uses per evaluation of each pass type
+    uses = {"cd06_x": 5, "cd06_y": 5, "cd06_z": 5, "cf90_x": 3, "cf90_y": 3, "cf90_z": 3, "poisson_5pass": 1}
+    res = {}
+    for name, fn in calls.items():
+        for _ in range(2):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name] = e0.elapsed_time(e1) / steps
+    share = {k: res[k] * uses[k] for k in res if k != "poisson_5pass"}
+    dom = max(share, key=share.get)
+    algo = 2 * N * 4  # read N words + write N words (FP32)
+    ach = algo / (res[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": PEAK_HBM, "unit": "GB/s",
+            "frac": round(ach / PEAK_HBM, 4), "traffic": None,
+            "bytes_per_launch": algo, "avg_launch_ms": round(res[dom], 5)}
+    return {k: round(v, 5) for k, v in res.items()}, roof
+
+
+def hbm_probe(steps):
+    """Plain device copy (1 GiB -> 1 GiB) and STREAM triad a = b + 2 c over a
+    4.3 GB working set (above the 256 MB last-level cache), timed with events."""
+    import torch
+    dev = torch.device("cuda:0")
+    out = {}
+    n = 1 << 28  # 1 GiB of FP32
+    a = torch.ones(n, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    c = torch.ones_like(a)
+    for name, fn, nbytes in (("copy", lambda: b.copy_(a), 2 * 4 * n),
+                             ("triad", lambda: torch.add(b, c, alpha=2.0, out=a), 3 * 4 * n)):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        out[name + "_GBps"] = round(nbytes / (ms * 1e-3) / 1e9, 1)
+    del a, b, c
+    return out
+
+
+def committed_traffic(kernel):
+    """HBM bytes per launch of the dominant pass, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+        return data.get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline():
+    """The oracle's FP64 full RHS (OpenMP C) on the host: one evaluation at
+    256^3, scaled by the point count to a 512^3 evaluation rate."""
+    import numpy as np
+
+    import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    n = 256
+    rng = np.random.default_rng(3)
+    u, v, w = (np.asfortranarray(rng.standard_normal((n, n, n))) for _ in range(3))
+    t0 = time.perf_counter()
+    O.full_rhs(u, v, w)
+    t = time.perf_counter() - t0
+    return {"value": round(1.0 / (t * 8.0), 5), "unit": "RHS evals/s (512^3-equivalent)",
+            "cores": O.num_threads(), "kind": "port",
+            "sample": "one FP64 full-RHS evaluation of the C oracle at 256^3 (%.1f s), rate scaled by 1/8 to "
+                      "512^3 points" % t}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,198 @@
+/*
+ * padeops_hip.h -- C ABI of the MI355X (gfx950) implementation of the PadeOps
+ * compact-derivative / compact-filter / periodic FFT-Poisson hot path.
+ *
+ * Every entry point is a plain C function over raw pointers and sizes so that
+ * the Fortran solvers can bind it with ISO_C_BINDING (see
+ * padeops_amd/fortran/padeops_hip_mod.F90).  Field arguments are DEVICE
+ * pointers laid out exactly like the reference's explicit-shape dummies
+ * (Fortran column-major, first index fastest):
+ *     x-operators  f(n, na, nb)      (dd1, d2d1, filter1)
+ *     y-operators  f(na, n, nb)      (dd2, d2d2, filter2)
+ *     z-operators  f(na, nb, n)      (dd3, d2d3, filter3, cd06stagg)
+ * Complex fields are interleaved (re, im) pairs; because every coefficient is
+ * real, a complex z-operator on (na, nb, n) equals the real operator on
+ * (2*na, nb, n).
+ *
+ * Calls are stream-ordered and asynchronous (stream = hipStream_t, NULL for
+ * the default stream) and never synchronize, except pdo_sync and the
+ * divergence maximum.  Objects hold only device-resident factor tables; the
+ * caller owns every field buffer.
+ *
+ * Return codes: 0 success; the reference's own init codes (2, 3, 7: too few
+ * points; 21: cd06stagg needs more than 4 points); 324 for a boundary flag
+ * outside {-1,0,1} (the reference aborts with GracefulExit code 324); negative
+ * values for errors of this library (pdo_last_error() gives the text).
+ */
+#ifndef PADEOPS_HIP_H
+#define PADEOPS_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* arithmetic/storage modes of an apply call */
+enum pdo_prec {
+    PDO_F64 = 0,        /* FP64 storage and arithmetic                               */
+    PDO_F32_REFINE = 1, /* FP32 storage; FP32 stencil and solves; one refinement
+                           step with the residual b - A y formed in FP64            */
+    PDO_F32_STORE = 2,  /* FP32 storage, FP64 arithmetic (no refinement needed)      */
+    PDO_F32 = 3         /* FP32 storage and arithmetic, no refinement (error study)  */
+};
+
+enum pdo_status {
+    PDO_OK = 0,
+    PDO_ERR_ARG = -1,   /* invalid argument (null handle, size mismatch, ...)        */
+    PDO_ERR_HIP = -2,   /* a HIP runtime call failed                                  */
+    PDO_ERR_UNSUP = -3, /* combination the reference leaves incomplete               */
+    PDO_ERR_NOGPU = -4  /* no HIP device                                              */
+};
+
+/* modified-wavenumber models for the Poisson solver (PoissonPeriodic%init
+ * Get_ModK callbacks, PoissonPeriodic.F90:114-212) */
+enum pdo_modk {
+    PDO_MODK_FOURIER = 0, /* exact wavenumbers (no callback)                          */
+    PDO_MODK_CD06 = 1,    /* collocated cd06 first derivative                         */
+    PDO_MODK_CD06STAGG = 2, /* staggered cd06, getmodCD06stagg PadeDerOps.F90:1034-1053 */
+    PDO_MODK_CD10 = 3     /* collocated cd10 first derivative                         */
+};
+
+typedef struct pdo_op_s *pdo_op;
+typedef struct pdo_poisson_s *pdo_poisson;
+typedef struct pdo_rhs_s *pdo_rhs;
+
+/* ---- library ------------------------------------------------------------ */
+const char *pdo_last_error(void);
+int pdo_device_count(int *count);
+int pdo_sync(void *stream);
+int pdo_malloc(void **ptr, size_t bytes);
+int pdo_free(void *ptr);
+int pdo_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int pdo_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+/* ---- cd06: 6th-order collocated compact derivative ----------------------
+ * replaces type(cd06) of src/derivatives/cd06.F90:
+ *   init        cd06.F90:129-207  (ierr 3: periodic n<6; 2: non-periodic n<8)
+ *   dd1/dd2/dd3 cd06.F90:775-839  (first derivative along x / y / z)
+ * d2d1/d2d2/d2d3: periodic second derivative (RHS cd06.F90:745-773 over
+ * dx^2, cyclic solve of the alpha = 2/11 system, cd06.F90:163-170); the
+ * reference leaves this operator unwired (derivatives.F90:524-525), so the
+ * non-periodic variant returns PDO_ERR_UNSUP.  Non-periodic cd06 supports
+ * bc1 = bcn = 0 only (bc 1 is incomplete in cd06.F90:288-292). */
+int pdo_cd06_init(pdo_op *h, int n, double dx, int periodic, int bc1, int bcn);
+int pdo_cd06_dd1(pdo_op h, int prec, const void *f, void *df, int na, int nb, void *stream);
+int pdo_cd06_dd2(pdo_op h, int prec, const void *f, void *df, int na, int nb, void *stream);
+int pdo_cd06_dd3(pdo_op h, int prec, const void *f, void *df, int na, int nb, void *stream);
+int pdo_cd06_d2d1(pdo_op h, int prec, const void *f, void *df, int na, int nb, void *stream);
+int pdo_cd06_d2d2(pdo_op h, int prec, const void *f, void *df, int na, int nb, void *stream);
+int pdo_cd06_d2d3(pdo_op h, int prec, const void *f, void *df, int na, int nb, void *stream);
+
+/* ---- cd10: 10th-order collocated compact derivatives ---------------------
+ * replaces type(cd10) of src/derivatives/cd10.F90:
+ *   init           cd10.F90:195-315  (ierr 2: n<8)
+ *   dd1/dd2/dd3    cd10.F90:2029-2237 (bc1/bcn in {0,1,-1}: one-sided closure,
+ *                  symmetric, antisymmetric; 324 otherwise)
+ *   d2d1/d2d2/d2d3 cd10.F90:2239-2447 */
+int pdo_cd10_init(pdo_op *h, int n, double dx, int periodic, int bc1, int bcn);
+int pdo_cd10_dd1(pdo_op h, int prec, const void *f, void *df, int na, int nb, int bc1, int bcn, void *stream);
+int pdo_cd10_dd2(pdo_op h, int prec, const void *f, void *df, int na, int nb, int bc1, int bcn, void *stream);
+int pdo_cd10_dd3(pdo_op h, int prec, const void *f, void *df, int na, int nb, int bc1, int bcn, void *stream);
+int pdo_cd10_d2d1(pdo_op h, int prec, const void *f, void *df, int na, int nb, int bc1, int bcn, void *stream);
+int pdo_cd10_d2d2(pdo_op h, int prec, const void *f, void *df, int na, int nb, int bc1, int bcn, void *stream);
+int pdo_cd10_d2d3(pdo_op h, int prec, const void *f, void *df, int na, int nb, int bc1, int bcn, void *stream);
+
+/* ---- cf90: 8th-order compact filter ---------------------------------------
+ * replaces type(cf90) of src/filters/cf90.F90:
+ *   init              cf90.F90:107-177 (ierr 7: n<10)
+ *   filter1/2/3       cf90.F90:1020-1224
+ *   filter3 (complex) cf90.F90:1226-1251; like the reference it always uses
+ *                     the non-periodic bc (0,0) system and its RHS for the
+ *                     given bc pair, so a periodic object returns PDO_ERR_UNSUP */
+int pdo_cf90_init(pdo_op *h, int n, int periodic);
+int pdo_cf90_filter1(pdo_op h, int prec, const void *f, void *fil, int na, int nb, int bc1, int bcn, void *stream);
+int pdo_cf90_filter2(pdo_op h, int prec, const void *f, void *fil, int na, int nb, int bc1, int bcn, void *stream);
+int pdo_cf90_filter3(pdo_op h, int prec, const void *f, void *fil, int na, int nb, int bc1, int bcn, void *stream);
+int pdo_cf90_filter3_cmplx(pdo_op h, int prec, const void *f, void *fil, int na, int nb, int bc1, int bcn,
+                           void *stream);
+
+/* ---- cd06stagg: staggered 6th-order z operators (periodic) ---------------
+ * replaces type(cd06stagg) of src/derivatives/cd06stagg.F90, periodic init
+ * (:170-195).  Cell fields have n points, edge fields n+1 (edge n+1 repeats
+ * edge 1).  is_complex != 0: interleaved complex (n1 complex points per row).
+ *   ddz_E2C :820-845  ddz_C2E :846-873  d2dz2_C2C :1008-1026
+ *   d2dz2_E2E :1027-1044  InterpZ_E2C :932-957  InterpZ_C2E :958-985 */
+int pdo_cd06stagg_init_periodic(pdo_op *h, int n, double dz);
+int pdo_cd06stagg_ddz_E2C(pdo_op h, int prec, const void *fE, void *dfC, int n1, int n2, int is_complex, void *stream);
+int pdo_cd06stagg_ddz_C2E(pdo_op h, int prec, const void *fC, void *dfE, int n1, int n2, int is_complex, void *stream);
+int pdo_cd06stagg_d2dz2_C2C(pdo_op h, int prec, const void *fC, void *d2fC, int n1, int n2, int is_complex,
+                            void *stream);
+int pdo_cd06stagg_d2dz2_E2E(pdo_op h, int prec, const void *fE, void *d2fE, int n1, int n2, int is_complex,
+                            void *stream);
+int pdo_cd06stagg_interpz_E2C(pdo_op h, int prec, const void *fE, void *ifC, int n1, int n2, int is_complex,
+                              void *stream);
+int pdo_cd06stagg_interpz_C2E(pdo_op h, int prec, const void *fC, void *ifE, int n1, int n2, int is_complex,
+                              void *stream);
+
+int pdo_op_destroy(pdo_op h);
+
+/* Host-side description of one operator variant (no device work): the dense
+ * system A y = B f it solves, A n x n and B n x n_in, row-major FP64.  kind:
+ * 0 first derivative, 1 second derivative, 2 filter, 10.. staggered
+ * (10 ddz_E2C, 11 ddz_C2E, 12 d2dz2_C2C, 13 d2dz2_E2E, 14 InterpZ_E2C,
+ * 15 InterpZ_C2E).  Used to check the tables against the oracle. */
+int pdo_op_dense(pdo_op h, int kind, int bc1, int bcn, double *A, double *B);
+/* Size query for pdo_op_dense: *n_sys rows, *n_in columns of B. */
+int pdo_op_dims(pdo_op h, int kind, int *n_sys, int *n_in);
+/* Partition chosen for an operator variant: segment length m and count P
+ * (P == 1 with m == n means the general single-pencil kernel). */
+int pdo_op_partition(pdo_op h, int kind, int bc1, int bcn, int *m, int *P);
+/* Copy of the partitioned-solve tables of a variant (host memory, FP64), for
+ * checking them on the CPU: meta = {m, P, K, nbl, nbr}; fac [4][m][5],
+ * spk [4][m][4], G [P][2K][2KP], sp [2][4][8].  general != 0 asks for the
+ * single-segment tables.  Pass NULL arrays to query meta only. */
+int pdo_op_tables(pdo_op h, int kind, int bc1, int bcn, int general, int *meta, double *fac, double *spk,
+                  double *G, double *sp);
+
+/* ---- periodic FFT Poisson solver -----------------------------------------
+ * replaces type(PoissonPeriodic), src/utilities/PoissonPeriodic.F90:
+ *   init          :114-212 (dir_id 1, x-decomposed input), modk_* choose the
+ *                 modified-wavenumber callback for each direction
+ *   poisson_solve :37-87 ; poisson3D_multiply :89-111
+ * zero_policy 0 reproduces the reference (divide by k^2+1e-20, zero only the
+ * (1,1,1) mode); 1 zeroes every mode with |k'|^2 <= 1e-14, as
+ * PadePoisson.F90:104-108 does (needed for collocated modified wavenumbers,
+ * which vanish at the Nyquist modes).  nx must be a power of two in [16, 1024],
+ * ny and nz powers of two in [8, 1024].  rhs and f are real (nx, ny, nz); rhs == f (in place) allowed. */
+int pdo_poisson_periodic_init(pdo_poisson *h, int nx, int ny, int nz, double dx, double dy, double dz, int modk_x,
+                              int modk_y, int modk_z, int zero_policy);
+int pdo_poisson_periodic_solve(pdo_poisson h, int prec, const void *rhs, void *f, void *stream);
+int pdo_poisson_destroy(pdo_poisson h);
+
+/* ---- full RHS evaluation (the benchmarked composite) ----------------------
+ * On a triply periodic n^3 grid of side L, for velocity (u, v, w):
+ *   grad(:,:,:,3*i+j) = d u_i / d x_j          9 cd06 first derivatives
+ *   ut_i = F_z F_y F_x u_i                      9 cf90 filter passes
+ *   div  = sum_j d ut_j / d x_j                 3 cd06 passes (accumulated)
+ *   p    = Poisson(div), cd06 modified wavenumbers, |k'|^2 <= 1e-14 zeroed
+ *   uo_i = ut_i - d p / d x_i                   3 cd06 passes
+ * built from derivatives%ddx/ddy/ddz (derivatives.F90:447-510),
+ * filters%filterx/y/z (filters.F90:220-269) and PoissonPeriodic.  grad holds 9
+ * consecutive fields.  prec: PDO_F64 or PDO_F32_REFINE / PDO_F32_STORE /
+ * PDO_F32 (all buffers then FP32).  Work buffers are allocated at init. */
+int pdo_rhs_init(pdo_rhs *h, int n, double L, int prec);
+int pdo_rhs_eval(pdo_rhs h, const void *u, const void *v, const void *w, void *grad, void *uo, void *vo, void *wo,
+                 void *stream);
+/* pressure of the last evaluation (device, real n^3) */
+int pdo_rhs_pressure(pdo_rhs h, const void **p);
+int pdo_rhs_destroy(pdo_rhs h);
+
+/* max |D.u| with the cd06 divergence (DivergenceCheck, PadePoisson.F90:1165-1244),
+ * synchronizes the stream; result on the host. */
+int pdo_rhs_max_divergence(pdo_rhs h, const void *u, const void *v, const void *w, double *maxdiv, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PADEOPS_HIP_H */

@@ -1,0 +1,315 @@
+"""MI355X-native PadeOps hot path: compact derivatives, compact filter and the
+periodic FFT Poisson projection, as hand-written gfx950 HIP kernels behind a C
+ABI (include/padeops_hip.h).
+
+This module mirrors the reference's Fortran type-bound API for the path:
+
+    reference (Fortran)                          here (Python over the C ABI)
+    type(cd06);    ierr = d%init(n,dx,per,b1,bn) d = cd06(); ierr = d.init(...)
+    call d%dd1(f, df, na, nb)                    d.dd1(f, df, na, nb)
+    type(cd10) dd1..3 / d2d1..3 (bc1, bcn)       cd10().dd1(f, df, na, nb, bc1, bcn)
+    type(cf90) filter1..3 (bc1, bcn)             cf90().filter1(f, fil, na, nb, bc1, bcn)
+    type(cd06stagg) ddz_E2C ... InterpZ_C2E      cd06stagg().ddz_E2C(fE, dfC, n1, n2)
+    type(PoissonPeriodic) poisson_solve          PoissonPeriodic().poisson_solve(rhs, f)
+
+Field arguments are torch tensors resident on the GPU (PyTorch supplies the
+device memory and stream only), holding the reference's column-major layout
+in a flat, contiguous buffer: f(n, na, nb) for x, f(na, n, nb) for y,
+f(na, nb, n) for z.  float64 tensors run the FP64 path; float32 tensors run
+the precision chosen by ``prec`` ("f32_refine" by default).
+
+Errors: the reference's init codes are returned (2, 3, 7, 21); a boundary
+flag outside {-1,0,1} raises GracefulExit(code=324) like the reference's
+GracefulExit(..., 324); library failures raise PdoError.
+"""
+import ctypes
+
+from . import _lib
+from ._lib import LibraryMissing, lib  # noqa: F401
+
+PDO_F64, PDO_F32_REFINE, PDO_F32_STORE, PDO_F32 = 0, 1, 2, 3
+PREC = {"f64": PDO_F64, "f32_refine": PDO_F32_REFINE, "f32_store": PDO_F32_STORE, "f32": PDO_F32}
+MODK = {"fourier": 0, "cd06": 1, "cd06stagg": 2, "cd10": 3}
+
+
+class PdoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("padeops_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+class GracefulExit(RuntimeError):
+    """Raised where the reference calls GracefulExit (an MPI abort there)."""
+
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
+
+
+def _check(rc):
+    if rc == 0:
+        return 0
+    msg = _lib.last_error()
+    if rc == 324:
+        raise GracefulExit(324, msg or "Incorrect boundary specification")
+    if rc > 0:
+        return rc
+    raise PdoError(rc, msg)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("field buffers must be GPU tensors")
+    if not t.is_contiguous():
+        raise ValueError("field buffers must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is not None:
+        return ctypes.c_void_p(stream)
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _prec(f, prec):
+    import torch
+    if f.dtype == torch.float64:
+        return PDO_F64
+    if f.dtype == torch.float32:
+        return PREC[prec] if prec != "f64" else PDO_F32_REFINE
+    raise TypeError("fields must be float64 or float32")
+
+
+def _check_sizes(f, out, n_in, n_out, na, nb):
+    if f.numel() < n_in * na * nb or out.numel() < n_out * na * nb:
+        raise ValueError("buffer smaller than the declared shape")
+    if f.dtype != out.dtype:
+        raise TypeError("input and output dtypes differ")
+
+
+class _Op:
+    _init_fn = None
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def _handle(self):
+        if not self._h:
+            raise PdoError(-1, "object not initialised")
+        return self._h
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_op_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def dense(self, kind, bc1=0, bcn=0):
+        """(A, B) dense system of one variant (host-side description, no GPU)."""
+        import numpy as np
+        ns, ni = ctypes.c_int(), ctypes.c_int()
+        _check(lib().pdo_op_dims(self._handle(), kind, ctypes.byref(ns), ctypes.byref(ni)))
+        A = np.zeros((ns.value, ns.value))
+        B = np.zeros((ns.value, ni.value))
+        dp = ctypes.POINTER(ctypes.c_double)
+        _check(lib().pdo_op_dense(self._handle(), kind, bc1, bcn, A.ctypes.data_as(dp), B.ctypes.data_as(dp)))
+        return A, B
+
+    def partition(self, kind, bc1=0, bcn=0):
+        m, P = ctypes.c_int(), ctypes.c_int()
+        _check(lib().pdo_op_partition(self._handle(), kind, bc1, bcn, ctypes.byref(m), ctypes.byref(P)))
+        return m.value, P.value
+
+
+def _apply(fn, h, f, df, na, nb, n_in, n_out, prec, stream, *bc):
+    _check_sizes(f, df, n_in, n_out, na, nb)
+    return _check(fn(h, _prec(f, prec), _ptr(f), _ptr(df), na, nb, *bc, _stream(stream)))
+
+
+class cd06(_Op):
+    """type(cd06), src/derivatives/cd06.F90."""
+
+    def init(self, n, dx, periodic, bc1=0, bcn=0):
+        self.n = n
+        return _check(lib().pdo_cd06_init(ctypes.byref(self._h), n, dx, int(periodic), bc1, bcn))
+
+    def dd1(self, f, df, na, nb, prec="f32_refine", stream=None):
+        return _apply(lib().pdo_cd06_dd1, self._handle(), f, df, na, nb, self.n, self.n, prec, stream)
+
+    def dd2(self, f, df, na, nb, prec="f32_refine", stream=None):
+        return _apply(lib().pdo_cd06_dd2, self._handle(), f, df, na, nb, self.n, self.n, prec, stream)
+
+    def dd3(self, f, df, na, nb, prec="f32_refine", stream=None):
+        return _apply(lib().pdo_cd06_dd3, self._handle(), f, df, na, nb, self.n, self.n, prec, stream)
+
+    def d2d1(self, f, df, na, nb, prec="f32_refine", stream=None):
+        return _apply(lib().pdo_cd06_d2d1, self._handle(), f, df, na, nb, self.n, self.n, prec, stream)
+
+    def d2d2(self, f, df, na, nb, prec="f32_refine", stream=None):
+        return _apply(lib().pdo_cd06_d2d2, self._handle(), f, df, na, nb, self.n, self.n, prec, stream)
+
+    def d2d3(self, f, df, na, nb, prec="f32_refine", stream=None):
+        return _apply(lib().pdo_cd06_d2d3, self._handle(), f, df, na, nb, self.n, self.n, prec, stream)
+
+
+class cd10(_Op):
+    """type(cd10), src/derivatives/cd10.F90."""
+
+    def init(self, n, dx, periodic, bc1=0, bcn=0):
+        self.n = n
+        return _check(lib().pdo_cd10_init(ctypes.byref(self._h), n, dx, int(periodic), bc1, bcn))
+
+    def _ap(self, name, f, df, na, nb, bc1, bcn, prec, stream):
+        return _apply(getattr(lib(), name), self._handle(), f, df, na, nb, self.n, self.n, prec, stream, bc1, bcn)
+
+    def dd1(self, f, df, na, nb, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_cd10_dd1", f, df, na, nb, bc1, bcn, prec, stream)
+
+    def dd2(self, f, df, na, nb, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_cd10_dd2", f, df, na, nb, bc1, bcn, prec, stream)
+
+    def dd3(self, f, df, na, nb, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_cd10_dd3", f, df, na, nb, bc1, bcn, prec, stream)
+
+    def d2d1(self, f, df, na, nb, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_cd10_d2d1", f, df, na, nb, bc1, bcn, prec, stream)
+
+    def d2d2(self, f, df, na, nb, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_cd10_d2d2", f, df, na, nb, bc1, bcn, prec, stream)
+
+    def d2d3(self, f, df, na, nb, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_cd10_d2d3", f, df, na, nb, bc1, bcn, prec, stream)
+
+
+class cf90(_Op):
+    """type(cf90), src/filters/cf90.F90."""
+
+    def init(self, n, periodic):
+        self.n = n
+        return _check(lib().pdo_cf90_init(ctypes.byref(self._h), n, int(periodic)))
+
+    def _ap(self, name, f, df, na, nb, bc1, bcn, prec, stream):
+        return _apply(getattr(lib(), name), self._handle(), f, df, na, nb, self.n, self.n, prec, stream, bc1, bcn)
+
+    def filter1(self, f, fil, na, nb, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_cf90_filter1", f, fil, na, nb, bc1, bcn, prec, stream)
+
+    def filter2(self, f, fil, na, nb, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_cf90_filter2", f, fil, na, nb, bc1, bcn, prec, stream)
+
+    def filter3(self, f, fil, na, nb, bc1=0, bcn=0, prec="f32_refine", stream=None, complex_=False):
+        name = "pdo_cf90_filter3_cmplx" if complex_ else "pdo_cf90_filter3"
+        k = 2 if complex_ else 1
+        _check_sizes(f, fil, self.n * k, self.n * k, na, nb)
+        return _check(getattr(lib(), name)(self._handle(), _prec(f, prec), _ptr(f), _ptr(fil), na, nb, bc1, bcn,
+                                           _stream(stream)))
+
+
+class cd06stagg(_Op):
+    """type(cd06stagg), src/derivatives/cd06stagg.F90 (periodic)."""
+
+    def init(self, n, dz):
+        self.n = n
+        return _check(lib().pdo_cd06stagg_init_periodic(ctypes.byref(self._h), n, dz))
+
+    def _ap(self, name, fi, fo, n1, n2, n_in, n_out, is_complex, prec, stream):
+        k = 2 if is_complex else 1
+        if fi.numel() < k * n1 * n2 * n_in or fo.numel() < k * n1 * n2 * n_out:
+            raise ValueError("buffer smaller than the declared shape")
+        return _check(getattr(lib(), name)(self._handle(), _prec(fi, prec), _ptr(fi), _ptr(fo), n1, n2,
+                                           int(is_complex), _stream(stream)))
+
+    def ddz_E2C(self, fE, dfC, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        return self._ap("pdo_cd06stagg_ddz_E2C", fE, dfC, n1, n2, self.n + 1, self.n, is_complex, prec, stream)
+
+    def ddz_C2E(self, fC, dfE, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        return self._ap("pdo_cd06stagg_ddz_C2E", fC, dfE, n1, n2, self.n, self.n + 1, is_complex, prec, stream)
+
+    def d2dz2_C2C(self, fC, d2fC, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        return self._ap("pdo_cd06stagg_d2dz2_C2C", fC, d2fC, n1, n2, self.n, self.n, is_complex, prec, stream)
+
+    def d2dz2_E2E(self, fE, d2fE, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        return self._ap("pdo_cd06stagg_d2dz2_E2E", fE, d2fE, n1, n2, self.n + 1, self.n + 1, is_complex, prec,
+                        stream)
+
+    def InterpZ_E2C(self, fE, ifC, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        return self._ap("pdo_cd06stagg_interpz_E2C", fE, ifC, n1, n2, self.n + 1, self.n, is_complex, prec, stream)
+
+    def InterpZ_C2E(self, fC, ifE, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        return self._ap("pdo_cd06stagg_interpz_C2E", fC, ifE, n1, n2, self.n, self.n + 1, is_complex, prec, stream)
+
+
+class PoissonPeriodic:
+    """type(PoissonPeriodic), src/utilities/PoissonPeriodic.F90 (dir_id = 1)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def init(self, dx, dy, dz, nx, ny, nz, modk=("fourier", "fourier", "fourier"), zero_policy=0):
+        self.shape = (nx, ny, nz)
+        mk = [MODK[m] if isinstance(m, str) else int(m) for m in modk]
+        return _check(lib().pdo_poisson_periodic_init(ctypes.byref(self._h), nx, ny, nz, dx, dy, dz, mk[0], mk[1],
+                                                      mk[2], zero_policy))
+
+    def poisson_solve(self, rhs, f=None, prec="f32_refine", stream=None):
+        """In place when f is None (poisson_solve_inplace), else out of place."""
+        out = rhs if f is None else f
+        n = self.shape[0] * self.shape[1] * self.shape[2]
+        if rhs.numel() < n or out.numel() < n:
+            raise ValueError("buffer smaller than the declared shape")
+        return _check(lib().pdo_poisson_periodic_solve(self._h, _prec(rhs, prec), _ptr(rhs), _ptr(out),
+                                                       _stream(stream)))
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_poisson_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class FullRHS:
+    """The benchmarked RHS evaluation (see pdo_rhs_eval in include/padeops_hip.h)."""
+
+    def __init__(self, n, L, prec="f64"):
+        self._h = ctypes.c_void_p()
+        self.n = n
+        self.prec = PREC[prec]
+        _check(lib().pdo_rhs_init(ctypes.byref(self._h), n, L, self.prec))
+
+    def eval(self, u, v, w, grad, uo, vo, wo, stream=None):
+        return _check(lib().pdo_rhs_eval(self._h, _ptr(u), _ptr(v), _ptr(w), _ptr(grad), _ptr(uo), _ptr(vo),
+                                         _ptr(wo), _stream(stream)))
+
+    def pressure_ptr(self):
+        p = ctypes.c_void_p()
+        _check(lib().pdo_rhs_pressure(self._h, ctypes.byref(p)))
+        return p.value
+
+    def max_divergence(self, u, v, w, stream=None):
+        d = ctypes.c_double()
+        _check(lib().pdo_rhs_max_divergence(self._h, _ptr(u), _ptr(v), _ptr(w), ctypes.byref(d), _stream(stream)))
+        return d.value
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_rhs_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -1,0 +1,96 @@
+"""Loader of the in-tree HIP library libpadeops_hip.so (ctypes).
+
+There is no CPU fallback: if the library is missing the import fails loudly,
+and every compute call needs a HIP device.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpadeops_hip.so")
+
+_lib = None
+
+# (name, argtypes) of every entry point declared in include/padeops_hip.h
+_c_int, _c_double, _vp, _sz = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
+_pvp = ctypes.POINTER(ctypes.c_void_p)
+_pint = ctypes.POINTER(ctypes.c_int)
+_pdbl = ctypes.POINTER(ctypes.c_double)
+
+_APPLY = [_vp, _c_int, _vp, _vp, _c_int, _c_int, _vp]
+_APPLY_BC = [_vp, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]
+_STAGG = [_vp, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _vp]
+
+SIGNATURES = {
+    "pdo_last_error": ([], ctypes.c_char_p),
+    "pdo_device_count": ([_pint], _c_int),
+    "pdo_sync": ([_vp], _c_int),
+    "pdo_malloc": ([_pvp, _sz], _c_int),
+    "pdo_free": ([_vp], _c_int),
+    "pdo_memcpy_h2d": ([_vp, _vp, _sz], _c_int),
+    "pdo_memcpy_d2h": ([_vp, _vp, _sz], _c_int),
+    "pdo_cd06_init": ([_pvp, _c_int, _c_double, _c_int, _c_int, _c_int], _c_int),
+    "pdo_cd06_dd1": (_APPLY, _c_int),
+    "pdo_cd06_dd2": (_APPLY, _c_int),
+    "pdo_cd06_dd3": (_APPLY, _c_int),
+    "pdo_cd06_d2d1": (_APPLY, _c_int),
+    "pdo_cd06_d2d2": (_APPLY, _c_int),
+    "pdo_cd06_d2d3": (_APPLY, _c_int),
+    "pdo_cd10_init": ([_pvp, _c_int, _c_double, _c_int, _c_int, _c_int], _c_int),
+    "pdo_cd10_dd1": (_APPLY_BC, _c_int),
+    "pdo_cd10_dd2": (_APPLY_BC, _c_int),
+    "pdo_cd10_dd3": (_APPLY_BC, _c_int),
+    "pdo_cd10_d2d1": (_APPLY_BC, _c_int),
+    "pdo_cd10_d2d2": (_APPLY_BC, _c_int),
+    "pdo_cd10_d2d3": (_APPLY_BC, _c_int),
+    "pdo_cf90_init": ([_pvp, _c_int, _c_int], _c_int),
+    "pdo_cf90_filter1": (_APPLY_BC, _c_int),
+    "pdo_cf90_filter2": (_APPLY_BC, _c_int),
+    "pdo_cf90_filter3": (_APPLY_BC, _c_int),
+    "pdo_cf90_filter3_cmplx": (_APPLY_BC, _c_int),
+    "pdo_cd06stagg_init_periodic": ([_pvp, _c_int, _c_double], _c_int),
+    "pdo_cd06stagg_ddz_E2C": (_STAGG, _c_int),
+    "pdo_cd06stagg_ddz_C2E": (_STAGG, _c_int),
+    "pdo_cd06stagg_d2dz2_C2C": (_STAGG, _c_int),
+    "pdo_cd06stagg_d2dz2_E2E": (_STAGG, _c_int),
+    "pdo_cd06stagg_interpz_E2C": (_STAGG, _c_int),
+    "pdo_cd06stagg_interpz_C2E": (_STAGG, _c_int),
+    "pdo_op_destroy": ([_vp], _c_int),
+    "pdo_op_dense": ([_vp, _c_int, _c_int, _c_int, _pdbl, _pdbl], _c_int),
+    "pdo_op_dims": ([_vp, _c_int, _pint, _pint], _c_int),
+    "pdo_op_partition": ([_vp, _c_int, _c_int, _c_int, _pint, _pint], _c_int),
+    "pdo_op_tables": ([_vp, _c_int, _c_int, _c_int, _c_int, _pint, _pdbl, _pdbl, _pdbl, _pdbl], _c_int),
+    "pdo_poisson_periodic_init": ([_pvp, _c_int, _c_int, _c_int, _c_double, _c_double, _c_double,
+                                   _c_int, _c_int, _c_int, _c_int], _c_int),
+    "pdo_poisson_periodic_solve": ([_vp, _c_int, _vp, _vp, _vp], _c_int),
+    "pdo_poisson_destroy": ([_vp], _c_int),
+    "pdo_rhs_init": ([_pvp, _c_int, _c_double, _c_int], _c_int),
+    "pdo_rhs_eval": ([_vp] + [_vp] * 7 + [_vp], _c_int),
+    "pdo_rhs_pressure": ([_vp, _pvp], _c_int),
+    "pdo_rhs_destroy": ([_vp], _c_int),
+    "pdo_rhs_max_divergence": ([_vp, _vp, _vp, _vp, _pdbl, _vp], _c_int),
+}
+
+
+class LibraryMissing(ImportError):
+    pass
+
+
+def lib():
+    """Load libpadeops_hip.so; raises LibraryMissing if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(
+                "%s is not built: run `python -m padeops_amd.build` (or __graft_entry__.build())" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().pdo_last_error().decode(errors="replace")

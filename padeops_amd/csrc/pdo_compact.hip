@@ -1,0 +1,691 @@
+// Fused compact-operator kernels for gfx950.
+//
+// One launch applies  y = A^-1 (B f)  along every pencil of a field, where B
+// is the explicit RHS stencil and A the tri/pentadiagonal (cyclic when
+// periodic) left-hand side of a PadeOps compact scheme.  The solve is a
+// partitioned ("spike") factorization: each pencil of n = P*m rows is cut into
+// P segments of m = 32 rows held in the registers of one lane; each lane
+//   1. evaluates the stencil for its rows from a register window of the input,
+//   2. solves its local banded block (no pivoting, factors precomputed in FP64),
+//   3. publishes the first and last K values of the local solution in LDS,
+//   4. reads the interface values of its neighbours from the precomputed
+//      inverse of the 2KP-unknown reduced system, and
+//   5. corrects its rows with the precomputed spike vectors.
+// Input is read once from HBM and the result written once: no intermediate
+// global traffic.  The FP32 refinement mode repeats steps 2-5 on the residual
+// b - A y formed in FP64 from the register window, so it costs no extra bytes.
+//
+// Layout of the two kernel shapes:
+//   strided (y and z pencils): a workgroup owns 64 consecutive pencils along
+//     the contiguous index (one lane each) and all P segments (one wave each),
+//     so every load/store instruction of a wave touches 64 consecutive values.
+//   x (contiguous pencils): a workgroup stages TP whole pencils in LDS with
+//     coalesced loads, lanes are (pencil, segment) pairs reading their window
+//     from LDS (segment rows padded by one element against bank conflicts),
+//     and results go back through LDS to coalesced stores.
+// Sizes outside the register path (n not a multiple of 32 or above 512) use a
+// one-lane-per-pencil kernel with global scratch, computing in FP64.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pdo_kernels.hpp"
+
+namespace pdo {
+
+template <int FAM> struct Fam;
+//                                  K  R  sigma shift center
+#define PDO_FAM(ID, K_, R_, S_, H_, C_)                                                   \
+    template <> struct Fam<ID> {                                                          \
+        static constexpr int K = K_, R = R_, SIGMA = S_, SHIFT = H_;                      \
+        static constexpr bool CENTER = C_;                                                \
+    };
+PDO_FAM(0, 1, 2, -1, 0, false)   // cd06 d1
+PDO_FAM(1, 1, 2, +1, 0, true)    // cd06 d2, staggered d2
+PDO_FAM(2, 2, 3, -1, 0, false)   // cd10 d1
+PDO_FAM(3, 2, 3, +1, 0, true)    // cd10 d2
+PDO_FAM(4, 2, 4, +1, 0, true)    // cf90
+PDO_FAM(5, 1, 2, -1, +1, false)  // staggered ddz E2C
+PDO_FAM(6, 1, 2, -1, -1, false)  // staggered ddz C2E
+PDO_FAM(7, 1, 2, +1, +1, false)  // staggered interp E2C
+PDO_FAM(8, 1, 2, +1, -1, false)  // staggered interp C2E
+#undef PDO_FAM
+
+template <int MODE> struct Mode;
+template <> struct Mode<MODE_F64> { using TS = double; using TC = double; static constexpr bool REFINE = false; };
+template <> struct Mode<MODE_F32R> { using TS = float; using TC = float; static constexpr bool REFINE = true; };
+template <> struct Mode<MODE_F32S> { using TS = float; using TC = double; static constexpr bool REFINE = false; };
+template <> struct Mode<MODE_F32> { using TS = float; using TC = float; static constexpr bool REFINE = false; };
+
+template <typename T> __device__ __forceinline__ const T *tab(const ApplyArgs &A);
+template <> __device__ __forceinline__ const double *tab<double>(const ApplyArgs &A) { return A.tab_d; }
+template <> __device__ __forceinline__ const float *tab<float>(const ApplyArgs &A) { return A.tab_f; }
+
+// interior stencil of output row j; window index of input row (row0 + j + o)
+// is j + R + o.
+template <int FAM, typename T, typename TW, int W>
+__device__ __forceinline__ T stencil_row(const TW (&win)[W], int j, const T *c)
+{
+    using F = Fam<FAM>;
+    T acc = F::CENTER ? c[0] * (T)win[j + F::R] : T(0);
+#pragma unroll
+    for (int q = 1; q <= F::R; ++q) {
+        const int hi = j + F::R + q + (F::SHIFT < 0 ? -1 : 0);
+        const int lo = j + F::R - q + (F::SHIFT > 0 ? 1 : 0);
+        const T pr = F::SIGMA > 0 ? (T)win[hi] + (T)win[lo] : (T)win[hi] - (T)win[lo];
+        acc = fma(c[q], pr, acc);
+    }
+    return acc;
+}
+
+// RHS row j of segment `seg` with the closure rows of non-periodic systems.
+// Closure rows are stored WBMAX wide: left rows against input columns
+// [0, WBMAX), right rows against [n_in - WBMAX, n_in), so every window index is
+// a compile-time constant (n_in == n_sys for all non-periodic systems).
+template <int FAM, typename T, typename TW, int W>
+__device__ __forceinline__ T rhs_row(const TW (&win)[W], int j, const T *c, const ApplyArgs &A, int ty, int m)
+{
+    using F = Fam<FAM>;
+    static_assert(MAXM + F::R - WBMAX >= 0, "closure window");
+    if (!A.periodic) {
+        if ((ty == 0 || ty == 3) && j < A.nbl) {
+            const double *sp = A.tab_d + A.o_sp + j * WBMAX;
+            T acc = 0;
+#pragma unroll
+            for (int q = 0; q < WBMAX; ++q) acc = fma((T)sp[q], (T)win[q + F::R], acc);
+            return acc;
+        }
+        if ((ty == 2 || ty == 3) && j >= m - A.nbr) {
+            const double *sp = A.tab_d + A.o_sp + (NBMAX + j - (m - A.nbr)) * WBMAX;
+            T acc = 0;
+#pragma unroll
+            for (int q = 0; q < WBMAX; ++q) acc = fma((T)sp[q], (T)win[MAXM + F::R - WBMAX + q], acc);
+            return acc;
+        }
+    }
+    return stencil_row<FAM, T>(win, j, c);
+}
+
+// local banded solve, in place, factors fac[j*5 + {l1,l2,inv,u1,u2}]
+template <int K, int MM, bool EXACT, typename T>
+__device__ __forceinline__ void local_solve(T (&d)[MM], const T *fac, int m)
+{
+#pragma unroll
+    for (int j = 1; j < MM; ++j) {
+        if (EXACT || j < m) {
+            T v = d[j] - fac[j * 5 + 0] * d[j - 1];
+            if (K == 2 && j >= 2) v -= fac[j * 5 + 1] * d[j - 2];
+            d[j] = v;
+        }
+    }
+#pragma unroll
+    for (int j = MM - 1; j >= 0; --j) {
+        if (EXACT || j < m) {
+            T v = d[j] * fac[j * 5 + 2];
+            if (j + 1 < MM && (EXACT || j + 1 < m)) v -= fac[j * 5 + 3] * d[j + 1];
+            if (K == 2 && j + 2 < MM && (EXACT || j + 2 < m)) v -= fac[j * 5 + 4] * d[j + 2];
+            d[j] = v;
+        }
+    }
+}
+
+template <int K, int MM, bool EXACT, typename T>
+__device__ __forceinline__ void apply_spikes(T (&d)[MM], const T *spk, const T (&yl)[2], const T (&yr)[2], int m)
+{
+#pragma unroll
+    for (int j = 0; j < MM; ++j) {
+        if (EXACT || j < m) {
+            T v = d[j];
+#pragma unroll
+            for (int q = 0; q < K; ++q) v -= spk[j * 4 + q] * yl[q] + spk[j * 4 + 2 + q] * yr[q];
+            d[j] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// strided pencils (y, z): NPW pencils x P segments per workgroup
+// ---------------------------------------------------------------------------
+// NPW = 64: one wave per segment (segment index uniform across the wave);
+// NPW = 32: two segments per wave (lanes 0-31 and 32-63), half the workgroup
+// size, so two workgroups share a CU and overlap their load, solve and store
+// phases.  PER (periodic) systems have identical segments: factors and spikes
+// are uniform, and the reduced inverse is block circulant, so the interface
+// values are sums over relative segment offsets d with uniform coefficients
+// G_0[., d] and per-lane LDS addresses.
+template <int FAM, int MODE, int NPW, bool PER>
+__device__ __forceinline__ void reduce_seg(const typename Mode<MODE>::TC *red, const ApplyArgs &A, int s, int pl,
+                                           typename Mode<MODE>::TC (&yl)[2], typename Mode<MODE>::TC (&yr)[2])
+{
+    using TC = typename Mode<MODE>::TC;
+    constexpr int K = Fam<FAM>::K;
+    const int P = A.P, NR = 2 * K * P;
+    yl[0] = yl[1] = yr[0] = yr[1] = TC(0);
+    if constexpr (PER) {
+        const TC *G = tab<TC>(A) + A.o_G;
+        for (int dd = 0; dd < P; ++dd) {
+            int t = s + dd;
+            t -= (t >= P) ? P : 0;
+#pragma unroll
+            for (int r = 0; r < 2 * K; ++r) {
+                const TC zv = red[(t * 2 * K + r) * NPW + pl];
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    yl[q] = fma(G[q * NR + dd * 2 * K + r], zv, yl[q]);
+                    yr[q] = fma(G[(K + q) * NR + dd * 2 * K + r], zv, yr[q]);
+                }
+            }
+        }
+    } else {
+        const TC *G = tab<TC>(A) + A.o_G + (size_t)s * 2 * K * NR;
+        for (int t = 0; t < P; ++t) {
+#pragma unroll
+            for (int r = 0; r < 2 * K; ++r) {
+                const TC zv = red[(t * 2 * K + r) * NPW + pl];
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    yl[q] = fma(G[q * NR + t * 2 * K + r], zv, yl[q]);
+                    yr[q] = fma(G[(K + q) * NR + t * 2 * K + r], zv, yr[q]);
+                }
+            }
+        }
+    }
+}
+
+template <int FAM, int MODE, int NPW, bool PER>
+__global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
+{
+    using TS = typename Mode<MODE>::TS;
+    using TC = typename Mode<MODE>::TC;
+    using F = Fam<FAM>;
+    constexpr int K = F::K, R = F::R, MM = MAXM, W = MM + 2 * R;
+    constexpr bool REFINE = Mode<MODE>::REFINE;
+    static_assert(NPW == 64 || PER, "two segments per wave need uniform (periodic) tables");
+    using TR = typename std::conditional<REFINE, double, TC>::type;  // RHS arithmetic
+    const int m = MM, P = A.P;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int pl = lane % NPW;
+    const int s = (NPW == 64) ? wv : wv * 2 + (lane >> 5);
+    const long a = (long)blockIdx.x * NPW + pl;
+    const bool act = a < A.na;
+    const int ty = PER ? 1 : (P == 1 ? 3 : (s == 0 ? 0 : (s == P - 1 ? 2 : 1)));
+
+    const TS *__restrict__ fp = (const TS *)A.f + ((act ? a : 0) + (long)blockIdx.y * A.sb_in);
+    const int se = (int)A.se_in;
+    TS win[W];
+    const int r0 = s * m - R;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        int r = r0 + w;
+        bool ok = true;
+        if (PER) r = r < 0 ? r + A.n_sys : (r >= A.n_read ? r - A.n_sys : r);
+        else ok = (r >= 0) && (r < A.n_in);
+        win[w] = ok ? fp[r * se] : TS(0);
+    }
+
+    TR cr[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) cr[q] = (TR)A.c[q];
+    TC d[MM];
+#pragma unroll
+    for (int j = 0; j < MM; ++j) {
+        if (PER) d[j] = (TC)stencil_row<FAM, TR>(win, j, cr);
+        else d[j] = (TC)rhs_row<FAM, TR>(win, j, cr, A, ty, m);
+    }
+
+    const TC *fac = tab<TC>(A) + A.o_fac + ty * m * 5;
+    const TC *spk = tab<TC>(A) + A.o_spk + ty * m * 4;
+    extern __shared__ unsigned char smem[];
+    TC *red = (TC *)smem;
+    TC yl[2], yr[2];
+
+    local_solve<K, MM, true>(d, fac, m);
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        red[(s * 2 * K + q) * NPW + pl] = d[q];
+        red[(s * 2 * K + K + q) * NPW + pl] = d[m - K + q];
+    }
+    __syncthreads();
+    reduce_seg<FAM, MODE, NPW, PER>(red, A, s, pl, yl, yr);
+    apply_spikes<K, MM, true>(d, spk, yl, yr, m);
+
+    if constexpr (REFINE) {
+        // residual r = B f - A y in FP64.  The neighbour rows it needs (rows
+        // -K..-1 and m..m+K-1) are taken from the values the neighbouring
+        // segments actually hold, published through LDS, so that the residual
+        // belongs to the stored solution and the correction removes its error.
+        TC *red2 = red + P * 2 * K * NPW;
+        TC *red3 = red2 + P * 2 * K * NPW;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            red3[(s * 2 * K + q) * NPW + pl] = d[q];
+            red3[(s * 2 * K + K + q) * NPW + pl] = d[m - K + q];
+        }
+        __syncthreads();
+        {
+            const int pv = s > 0 ? s - 1 : (PER ? P - 1 : -1);
+            const int nx = s < P - 1 ? s + 1 : (PER ? 0 : -1);
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                yl[q] = pv >= 0 ? red3[(pv * 2 * K + K + q) * NPW + pl] : TC(0);
+                yr[q] = nx >= 0 ? red3[(nx * 2 * K + q) * NPW + pl] : TC(0);
+            }
+        }
+        const double *band = A.tab_d + A.o_lhs + ty * m * 5;
+        TC e[MM];
+#pragma unroll
+        for (int j = 0; j < MM; ++j) {
+            double acc = PER ? stencil_row<FAM, double>(win, j, cr) : rhs_row<FAM, double>(win, j, cr, A, ty, m);
+#pragma unroll
+            for (int o = -K; o <= K; ++o) {
+                const int jj = j + o;
+                const double yv = jj < 0 ? (double)yl[K + jj]
+                                         : (jj >= m ? (double)yr[jj - m] : (double)d[jj < 0 ? 0 : (jj >= MM ? MM - 1 : jj)]);
+                acc = fma(-band[j * 5 + o + 2], yv, acc);
+            }
+            e[j] = (TC)acc;
+        }
+        local_solve<K, MM, true>(e, fac, m);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            red2[(s * 2 * K + q) * NPW + pl] = e[q];
+            red2[(s * 2 * K + K + q) * NPW + pl] = e[m - K + q];
+        }
+        __syncthreads();
+        TC el[2], er[2];
+        reduce_seg<FAM, MODE, NPW, PER>(red2, A, s, pl, el, er);
+        apply_spikes<K, MM, true>(e, spk, el, er, m);
+#pragma unroll
+        for (int j = 0; j < MM; ++j) d[j] += e[j];
+    }
+
+    if (!act) return;
+    TS *__restrict__ op = (TS *)A.out + (a + (long)blockIdx.y * A.sb_out);
+    const TS *__restrict__ ap = (const TS *)A.aux + (A.aux ? a + (long)blockIdx.y * A.sb_out : 0);
+    const int so = (int)A.se_out;
+#pragma unroll
+    for (int j = 0; j < MM; ++j) {
+        const int idx = (s * m + j) * so;
+        TC v = d[j];
+        if (A.epi == 1) v = (TC)ap[idx] + v;
+        else if (A.epi == 2) v = (TC)ap[idx] - v;
+        op[idx] = (TS)v;
+    }
+    if (A.n_out > A.n_sys && s == 0) {
+        const int idx = A.n_sys * so;
+        TC v = d[0];
+        if (A.epi == 1) v = (TC)ap[idx] + v;
+        else if (A.epi == 2) v = (TC)ap[idx] - v;
+        op[idx] = (TS)v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// contiguous pencils (x): TP pencils staged in LDS, lanes = (pencil, segment)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int padrow(int r) { return r + (r >> 5); }  // one pad element per 32 rows
+
+template <int FAM, int MODE>
+__global__ void __launch_bounds__(256) k_xdir(ApplyArgs A, int TP, int LD)
+{
+    // periodic pencils only: every segment has the same factors (type 1)
+    extern __shared__ unsigned char smem[];
+    constexpr bool PER = true;
+    using TS = typename Mode<MODE>::TS;
+    using TC = typename Mode<MODE>::TC;
+    using F = Fam<FAM>;
+    constexpr int K = F::K, R = F::R, MM = MAXM, W = MM + 2 * R;
+    constexpr bool REFINE = Mode<MODE>::REFINE;
+    using TR = typename std::conditional<REFINE, double, TC>::type;
+    const int m = MM, P = A.P, n = A.n_sys;
+    const int tid = threadIdx.x;
+    const int s = tid % P, p = tid / P;
+    const long pen0 = (long)blockIdx.x * TP;
+    const long npen = A.na * A.nb;
+    const int tpa = (int)((npen - pen0) < TP ? (npen - pen0) : TP);  // pencils of this workgroup
+    TS *tile = (TS *)smem;
+    TC *red = (TC *)(smem + (size_t)TP * LD * sizeof(TS));
+
+    // coalesced staging of tpa contiguous pencils: 16-byte loads, pencil and
+    // row from shifts (n_in is 32 * 2^k on this path)
+    using V = typename std::conditional<sizeof(TS) == 8, double2, float4>::type;
+    constexpr int VW = 16 / sizeof(TS);
+    const int lgn = A.lgn;
+    const TS *__restrict__ f = (const TS *)A.f + pen0 * (long)A.n_in;
+    const int nvec = (tpa * A.n_in) / VW;
+#pragma unroll 8
+    for (int v = tid; v < nvec; v += 256) {
+        const V x = reinterpret_cast<const V *>(f)[v];
+        const int e = v * VW, pp = e >> lgn, r = e & (A.n_in - 1);
+        TS *dst = tile + pp * LD + padrow(r);
+        const TS *xs = reinterpret_cast<const TS *>(&x);
+#pragma unroll
+        for (int i = 0; i < VW; ++i) dst[i] = xs[i];
+    }
+    __syncthreads();
+
+    const bool act = p < tpa;
+    const int pc = act ? p : 0;
+    const int ty = PER ? 1 : (P == 1 ? 3 : (s == 0 ? 0 : (s == P - 1 ? 2 : 1)));
+    TS win[W];
+    const int r0 = s * m - R;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        int r = r0 + w;
+        bool ok = true;
+        if (PER) r = r < 0 ? r + n : (r >= n ? r - n : r);
+        else ok = (r >= 0) && (r < A.n_in);
+        win[w] = ok ? tile[pc * LD + padrow(ok ? r : 0)] : TS(0);
+    }
+    TR cr[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) cr[q] = (TR)A.c[q];
+    TC d[MM];
+#pragma unroll
+    for (int j = 0; j < MM; ++j) d[j] = (TC)rhs_row<FAM, TR>(win, j, cr, A, ty, m);
+
+    const TC *fac = tab<TC>(A) + A.o_fac + ty * m * 5;
+    const TC *spk = tab<TC>(A) + A.o_spk + ty * m * 4;
+    const int NR = 2 * K * P;
+    auto exchange = [&](TC *rb, TC (&v)[MM], TC (&yl)[2], TC (&yr)[2]) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            rb[(p * P + s) * 2 * K + q] = v[q];
+            rb[(p * P + s) * 2 * K + K + q] = v[m - K + q];
+        }
+        __syncthreads();
+        yl[0] = yl[1] = yr[0] = yr[1] = TC(0);
+        for (int dd = 0; dd < P; ++dd) {
+            // periodic systems are block circulant: G_s[.][t] = G_0[.][t - s]
+            const int t = PER ? (s + dd < P ? s + dd : s + dd - P) : dd;
+            const TC *G = tab<TC>(A) + A.o_G + (PER ? 0 : (size_t)s * 2 * K * NR);
+            const int col = PER ? dd : t;
+#pragma unroll
+            for (int r = 0; r < 2 * K; ++r) {
+                const TC zv = rb[(p * P + t) * 2 * K + r];
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    yl[q] = fma(G[q * NR + col * 2 * K + r], zv, yl[q]);
+                    yr[q] = fma(G[(K + q) * NR + col * 2 * K + r], zv, yr[q]);
+                }
+            }
+        }
+    };
+
+    TC yl[2], yr[2];
+    local_solve<K, MM, true>(d, fac, m);
+    exchange(red, d, yl, yr);
+    apply_spikes<K, MM, true>(d, spk, yl, yr, m);
+    if constexpr (REFINE) {
+        // neighbour rows of the residual from the values the neighbours hold
+        TC *rb3 = red + (size_t)2 * TP * P * 2 * K;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            rb3[(p * P + s) * 2 * K + q] = d[q];
+            rb3[(p * P + s) * 2 * K + K + q] = d[m - K + q];
+        }
+        __syncthreads();
+        {
+            const int pv = s > 0 ? s - 1 : P - 1;
+            const int nx = s < P - 1 ? s + 1 : 0;
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                yl[q] = rb3[(p * P + pv) * 2 * K + K + q];
+                yr[q] = rb3[(p * P + nx) * 2 * K + q];
+            }
+        }
+        const double *band = A.tab_d + A.o_lhs + ty * m * 5;
+        TC e[MM];
+#pragma unroll
+        for (int j = 0; j < MM; ++j) {
+            double acc = rhs_row<FAM, double>(win, j, cr, A, ty, m);
+#pragma unroll
+            for (int o = -K; o <= K; ++o) {
+                const int jj = j + o;
+                const double yv = jj < 0 ? (double)yl[K + jj] : (jj >= m ? (double)yr[jj - m] : (double)d[jj < 0 ? 0 : (jj >= MM ? MM - 1 : jj)]);
+                acc = fma(-band[j * 5 + o + 2], yv, acc);
+            }
+            e[j] = (TC)acc;
+        }
+        local_solve<K, MM, true>(e, fac, m);
+        TC el[2], er[2];
+        exchange(red + (size_t)TP * P * 2 * K, e, el, er);
+        apply_spikes<K, MM, true>(e, spk, el, er, m);
+#pragma unroll
+        for (int j = 0; j < MM; ++j) d[j] += e[j];
+    }
+    __syncthreads();  // every window read of the tile is done
+    if (act) {
+#pragma unroll
+        for (int j = 0; j < MM; ++j) tile[p * LD + padrow(s * m + j)] = (TS)d[j];
+    }
+    __syncthreads();
+    TS *__restrict__ out = (TS *)A.out + pen0 * (long)A.n_out;
+    const TS *__restrict__ aux = (const TS *)A.aux + (A.aux ? pen0 * (long)A.n_out : 0);
+#pragma unroll 8
+    for (int v = tid; v < nvec; v += 256) {
+        const int e = v * VW, pp = e >> lgn, r = e & (A.n_in - 1);
+        const TS *src = tile + pp * LD + padrow(r);
+        V x;
+        TS *xs = reinterpret_cast<TS *>(&x);
+#pragma unroll
+        for (int i = 0; i < VW; ++i) xs[i] = src[i];
+        if (A.epi) {
+            const V y = reinterpret_cast<const V *>(aux)[v];
+            const TS *ys = reinterpret_cast<const TS *>(&y);
+#pragma unroll
+            for (int i = 0; i < VW; ++i) {
+                const TC t = A.epi == 1 ? (TC)ys[i] + (TC)xs[i] : (TC)ys[i] - (TC)xs[i];
+                xs[i] = (TS)t;
+            }
+        }
+        reinterpret_cast<V *>(out)[v] = x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// general kernel: one lane per pencil, any n, any strides; FP64 arithmetic,
+// intermediate solution in a global FP64 scratch laid out [row][pencil]
+// ---------------------------------------------------------------------------
+template <int FAM, typename TS>
+__global__ void __launch_bounds__(256) k_general(ApplyArgs A, double *scratch)
+{
+    using F = Fam<FAM>;
+    constexpr int K = F::K, R = F::R;
+    const long npen = A.na * A.nb;
+    const long pid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= npen) return;
+    const long ia = pid % A.na, ib = pid / A.na;
+    const long pin = ia * A.sa_in + ib * A.sb_in;
+    const long po = ia * A.sa_out + ib * A.sb_out;
+    const int n = A.n_sys;
+    const TS *__restrict__ f = (const TS *)A.f;
+    const double *fac = A.tab_d + A.o_fac + (A.periodic ? 1 : 3) * n * 5;
+    const double *spk = A.tab_d + A.o_spk + (A.periodic ? 1 : 3) * n * 4;
+    auto in = [&](int r) -> double {
+        if (A.periodic) r = r < 0 ? r + n : (r >= A.n_read ? r - n : r);
+        else if (r < 0 || r >= A.n_in) return 0.0;
+        return (double)f[pin + (long)r * A.se_in];
+    };
+    auto scr = [&](int r) -> double & { return scratch[(long)r * npen + pid]; };
+    double w1 = 0, w2 = 0;
+    for (int i = 0; i < n; ++i) {
+        double d;
+        if (!A.periodic && i < A.nbl) {
+            d = 0;
+            for (int q = 0; q < WBMAX; ++q) d += A.tab_d[A.o_sp + i * WBMAX + q] * in(q);
+        } else if (!A.periodic && i >= n - A.nbr) {
+            d = 0;
+            const int jr = i - (n - A.nbr);
+            for (int q = 0; q < WBMAX; ++q) d += A.tab_d[A.o_sp + (NBMAX + jr) * WBMAX + q] * in(A.n_in - WBMAX + q);
+        } else {
+            d = F::CENTER ? A.c[0] * in(i) : 0.0;
+            for (int q = 1; q <= R; ++q) {
+                const double hi = in(i + q + (F::SHIFT < 0 ? -1 : 0));
+                const double lo = in(i - q + (F::SHIFT > 0 ? 1 : 0));
+                d += A.c[q] * (F::SIGMA > 0 ? hi + lo : hi - lo);
+            }
+        }
+        double v = d;
+        if (i >= 1) v -= fac[i * 5 + 0] * w1;
+        if (K == 2 && i >= 2) v -= fac[i * 5 + 1] * w2;
+        scr(i) = v;
+        w2 = w1;
+        w1 = v;
+    }
+    double z1 = 0, z2 = 0;
+    for (int i = n - 1; i >= 0; --i) {
+        double v = scr(i) * fac[i * 5 + 2];
+        if (i + 1 < n) v -= fac[i * 5 + 3] * z1;
+        if (K == 2 && i + 2 < n) v -= fac[i * 5 + 4] * z2;
+        scr(i) = v;
+        z2 = z1;
+        z1 = v;
+    }
+    double yl[2] = {0, 0}, yr[2] = {0, 0};
+    if (A.periodic) {
+        const int NR = 2 * K;
+        double zh[4];
+        for (int q = 0; q < K; ++q) { zh[q] = scr(q); zh[K + q] = scr(n - K + q); }
+        for (int q = 0; q < K; ++q)
+            for (int r = 0; r < NR; ++r) {
+                yl[q] += A.tab_d[A.o_G + q * NR + r] * zh[r];
+                yr[q] += A.tab_d[A.o_G + (K + q) * NR + r] * zh[r];
+            }
+    }
+    TS *__restrict__ out = (TS *)A.out;
+    const TS *__restrict__ aux = (const TS *)A.aux;
+    double first = 0;
+    for (int i = 0; i < n; ++i) {
+        double v = scr(i);
+        for (int q = 0; q < K; ++q) v -= spk[i * 4 + q] * yl[q] + spk[i * 4 + 2 + q] * yr[q];
+        if (i == 0) first = v;
+        const long idx = po + (long)i * A.se_out;
+        if (A.epi == 1) v = (double)aux[idx] + v;
+        else if (A.epi == 2) v = (double)aux[idx] - v;
+        out[idx] = (TS)v;
+    }
+    if (A.n_out > n) {
+        const long idx = po + (long)n * A.se_out;
+        double v = first;
+        if (A.epi == 1) v = (double)aux[idx] + v;
+        else if (A.epi == 2) v = (double)aux[idx] - v;
+        out[idx] = (TS)v;
+    }
+}
+
+__global__ void k_zero_words(unsigned *p, long words)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (long)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+hipError_t launch_zero_words(void *p, long words, hipStream_t st)
+{
+    if (words <= 0) return hipSuccess;
+    long blocks = (words + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)blocks), dim3(256), 0, st, (unsigned *)p, words);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <int FAM, int MODE, int NPW, bool PER>
+static hipError_t launch_strided_k(const ApplyArgs &A, hipStream_t st)
+{
+    using TC = typename Mode<MODE>::TC;
+    constexpr int K = Fam<FAM>::K;
+    dim3 grid((unsigned)((A.na + NPW - 1) / NPW), (unsigned)A.nb);
+    dim3 block(NPW * A.P);
+    size_t lds = (size_t)A.P * 2 * K * NPW * sizeof(TC) * (Mode<MODE>::REFINE ? 3 : 1);
+    hipLaunchKernelGGL((k_strided<FAM, MODE, NPW, PER>), grid, block, lds, st, A);
+    return hipGetLastError();
+}
+
+template <int FAM, int MODE>
+static hipError_t launch_strided_t(const ApplyArgs &A, hipStream_t st)
+{
+    if (!A.periodic) return launch_strided_k<FAM, MODE, 64, false>(A, st);
+    if (A.P % 2 == 0) return launch_strided_k<FAM, MODE, 32, true>(A, st);
+    return launch_strided_k<FAM, MODE, 64, true>(A, st);
+}
+
+template <int FAM, int MODE>
+static hipError_t launch_x_t(const ApplyArgs &A, hipStream_t st)
+{
+    using TS = typename Mode<MODE>::TS;
+    using TC = typename Mode<MODE>::TC;
+    constexpr int K = Fam<FAM>::K;
+    const int P = A.P;
+    int TP = 256 / P;
+    const int LD = A.n_in + (A.n_in >> 5) + 1;
+    auto lds_of = [&](int tp) {
+        return (size_t)tp * LD * sizeof(TS) + (size_t)tp * P * 2 * K * sizeof(TC) * (Mode<MODE>::REFINE ? 3 : 1);
+    };
+    while (TP > 1 && lds_of(TP) > 80 * 1024) TP >>= 1;
+    if (TP * P != 256) return hipErrorInvalidValue;
+    const long npen = A.na * A.nb;
+    dim3 grid((unsigned)((npen + TP - 1) / TP));
+    dim3 block(TP * P);
+    hipLaunchKernelGGL((k_xdir<FAM, MODE>), grid, block, lds_of(TP), st, A, TP, LD);
+    return hipGetLastError();
+}
+
+template <int FAM, int MODE>
+static hipError_t launch_general_t(const ApplyArgs &A, double *scratch, hipStream_t st)
+{
+    using TS = typename Mode<MODE>::TS;
+    const long npen = A.na * A.nb;
+    dim3 grid((unsigned)((npen + 255) / 256));
+    hipLaunchKernelGGL((k_general<FAM, TS>), grid, dim3(256), 0, st, A, scratch);
+    return hipGetLastError();
+}
+
+template <int FAM>
+static hipError_t dispatch_mode(int shape, int mode, const ApplyArgs &A, double *scratch, hipStream_t st)
+{
+    switch (shape) {
+    case SHAPE_STRIDED:
+        switch (mode) {
+        case MODE_F64: return launch_strided_t<FAM, MODE_F64>(A, st);
+        case MODE_F32R: return launch_strided_t<FAM, MODE_F32R>(A, st);
+        case MODE_F32S: return launch_strided_t<FAM, MODE_F32S>(A, st);
+        case MODE_F32: return launch_strided_t<FAM, MODE_F32>(A, st);
+        }
+        break;
+    case SHAPE_X:
+        if constexpr (Fam<FAM>::SHIFT == 0) {
+            switch (mode) {
+            case MODE_F64: return launch_x_t<FAM, MODE_F64>(A, st);
+            case MODE_F32R: return launch_x_t<FAM, MODE_F32R>(A, st);
+            case MODE_F32S: return launch_x_t<FAM, MODE_F32S>(A, st);
+            case MODE_F32: return launch_x_t<FAM, MODE_F32>(A, st);
+            }
+        }
+        break;
+    case SHAPE_GENERAL:
+        if (mode == MODE_F64) return launch_general_t<FAM, MODE_F64>(A, scratch, st);
+        return launch_general_t<FAM, MODE_F32S>(A, scratch, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_compact(int family, int shape, int mode, const ApplyArgs &A, double *scratch, hipStream_t st)
+{
+    switch (family) {
+    case 0: return dispatch_mode<0>(shape, mode, A, scratch, st);
+    case 1: return dispatch_mode<1>(shape, mode, A, scratch, st);
+    case 2: return dispatch_mode<2>(shape, mode, A, scratch, st);
+    case 3: return dispatch_mode<3>(shape, mode, A, scratch, st);
+    case 4: return dispatch_mode<4>(shape, mode, A, scratch, st);
+    case 5: return dispatch_mode<5>(shape, mode, A, scratch, st);
+    case 6: return dispatch_mode<6>(shape, mode, A, scratch, st);
+    case 7: return dispatch_mode<7>(shape, mode, A, scratch, st);
+    case 8: return dispatch_mode<8>(shape, mode, A, scratch, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace pdo

@@ -1,0 +1,429 @@
+// Periodic FFT Poisson solver kernels for gfx950.
+//
+// A real field f(nx, ny, nz) (x fastest) is transformed in five passes, each
+// one HBM read and one HBM write of the half spectrum (nx/2+1, ny, nz):
+//   1. x   real-to-complex: nx/2-point complex FFT of the packed line of
+//          (even, odd) samples, then the split step to the nx/2+1 modes
+//   2. y   complex FFT of every y line
+//   3. z   complex FFT, multiply by -1/|k'|^2 (modified wavenumbers), inverse
+//          complex FFT -- one pass, the line never leaves LDS
+//   4. y   inverse complex FFT
+//   5. x   complex-to-real (inverse split + nx/2-point inverse FFT), scaled by
+//          1/(nx ny nz)
+// Each pass runs a mixed radix-8/4/2 Stockham FFT on a tile of lines: the
+// first stage reads its operands straight from HBM, the intermediate stages
+// go through LDS, and the last stage writes straight to HBM, so a 512-point
+// line costs two LDS round trips.  Every thread holds exactly 8 complex
+// values per stage (one radix-8, two radix-4 or four radix-2 butterflies).
+// For y and z the tile is TX consecutive lines, so each global row access is
+// a contiguous run of TX complex values; twiddles come from a table.
+#include <hip/hip_runtime.h>
+
+#include "pdo_fft.hpp"
+
+namespace pdo {
+
+template <typename T> struct C2 { T x, y; };
+
+template <typename T> __device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T> __device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T> __device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return {a.x - b.x, a.y - b.y}; }
+// multiply by SIGN*i
+template <typename T, int SIGN> __device__ __forceinline__ C2<T> mul_i(C2<T> a)
+{
+    return SIGN > 0 ? C2<T>{-a.y, a.x} : C2<T>{a.y, -a.x};
+}
+
+template <typename T, int SIGN> __device__ __forceinline__ void dft2(C2<T> &a, C2<T> &b)
+{
+    const C2<T> t = csub(a, b);
+    a = cadd(a, b);
+    b = t;
+}
+
+// y_m = sum_r x_r (SIGN i)^(r m), in place, on x[o], x[o+s], x[o+2s], x[o+3s]
+template <typename T, int SIGN> __device__ __forceinline__ void dft4(C2<T> &x0, C2<T> &x1, C2<T> &x2, C2<T> &x3)
+{
+    const C2<T> a0 = cadd(x0, x2), a1 = csub(x0, x2);
+    const C2<T> b0 = cadd(x1, x3), b1 = mul_i<T, SIGN>(csub(x1, x3));
+    x0 = cadd(a0, b0);
+    x1 = cadd(a1, b1);
+    x2 = csub(a0, b0);
+    x3 = csub(a1, b1);
+}
+
+// 8-point DFT with exp(SIGN 2 pi i / 8): radix-2 split into two DFT4
+template <typename T, int SIGN> __device__ __forceinline__ void dft8(C2<T> (&v)[8])
+{
+    const T h = (T)0.70710678118654752440084436210484903928;
+    C2<T> a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        a[k] = cadd(v[k], v[k + 4]);
+        b[k] = csub(v[k], v[k + 4]);
+    }
+    // b_k *= W8^k
+    b[1] = cmul(b[1], C2<T>{h, SIGN * h});
+    b[2] = mul_i<T, SIGN>(b[2]);
+    b[3] = cmul(b[3], C2<T>{-h, SIGN * h});
+    dft4<T, SIGN>(a[0], a[1], a[2], a[3]);
+    dft4<T, SIGN>(b[0], b[1], b[2], b[3]);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        v[2 * m] = a[m];
+        v[2 * m + 1] = b[m];
+    }
+}
+
+// One Stockham stage of radix R on one line of length L, current sub-size Ns.
+// The thread owns base index jb in [0, L/8) and handles items
+// j = jb + it*(L/8), it < 8/R.  LD(i) reads element i of the stage input,
+// ST(i, v) writes element i of the stage output.  The caller synchronizes.
+template <typename T, int R, int SIGN> struct Stage {
+    template <class LD>
+    __device__ __forceinline__ static void load(C2<T> (&v)[8], LD ld, const C2<T> *__restrict__ tw, int L, int Ns,
+                                                int jb)
+    {
+        const int Q = L / R, L8 = L >> 3;
+#pragma unroll
+        for (int it = 0; it < 8 / R; ++it) {
+            const int j = jb + it * L8;
+            const int k = j & (Ns - 1);
+            const int step = L / (Ns * R);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                C2<T> x = ld(j + r * Q);
+                if (r > 0) {
+                    C2<T> w = tw[(k * r * step) & (L - 1)];
+                    if (SIGN > 0) w.y = -w.y;
+                    x = cmul(x, w);
+                }
+                v[it * R + r] = x;
+            }
+        }
+    }
+    __device__ __forceinline__ static void butterfly(C2<T> (&v)[8])
+    {
+        if constexpr (R == 8) {
+            dft8<T, SIGN>(v);
+        } else if constexpr (R == 4) {
+            dft4<T, SIGN>(v[0], v[1], v[2], v[3]);
+            dft4<T, SIGN>(v[4], v[5], v[6], v[7]);
+        } else {
+#pragma unroll
+            for (int it = 0; it < 4; ++it) dft2<T, SIGN>(v[2 * it], v[2 * it + 1]);
+        }
+    }
+    template <class STO>
+    __device__ __forceinline__ static void store(const C2<T> (&v)[8], STO st, int L, int Ns, int jb)
+    {
+        const int L8 = L >> 3;
+#pragma unroll
+        for (int it = 0; it < 8 / R; ++it) {
+            const int j = jb + it * L8;
+            const int k = j & (Ns - 1);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) st(base + r * Ns, v[it * R + r]);
+        }
+    }
+};
+
+// Full FFT of one line: the first stage reads through LDG, the last stage
+// writes through STG, the intermediate stages live in the LDS line accessor
+// `buf` (element i at buf(i)).  SRC_LDS: LDG reads the same LDS buffer, so the
+// first stage also waits for every read before writing.  All threads of the
+// block must call this (it synchronizes); threads with act == false only take
+// part in the barriers.
+template <typename T, int SIGN, bool SRC_LDS, int L, int NS, class LDG, class STG, class BUF>
+__device__ __forceinline__ void line_fft_stage(LDG &ldg, STG &stg, BUF &buf, const C2<T> *__restrict__ tw, int jb,
+                                               bool act)
+{
+    constexpr int R = (L / NS) >= 8 ? 8 : (L / NS);
+    constexpr bool first = (NS == 1), last = (NS * R == L);
+    C2<T> v[8];
+    if (act) {
+        if constexpr (first) {
+            Stage<T, R, SIGN>::load(v, ldg, tw, L, NS, jb);
+        } else {
+            auto lds = [&](int i) { return buf(i); };
+            Stage<T, R, SIGN>::load(v, lds, tw, L, NS, jb);
+        }
+        Stage<T, R, SIGN>::butterfly(v);
+    }
+    if (!first || SRC_LDS) __syncthreads();  // every LDS read of this stage is done
+    if (act) {
+        if constexpr (last) {
+            Stage<T, R, SIGN>::store(v, stg, L, NS, jb);
+        } else {
+            auto sts = [&](int i, C2<T> x) { buf(i) = x; };
+            Stage<T, R, SIGN>::store(v, sts, L, NS, jb);
+        }
+    }
+    if constexpr (!last) {
+        __syncthreads();  // stage output visible
+        line_fft_stage<T, SIGN, SRC_LDS, L, NS * R>(ldg, stg, buf, tw, jb, act);
+    }
+}
+
+// Full FFT of one line of L points: the first stage reads through LDG, the
+// last stage writes through STG, the intermediate stages live in the LDS line
+// accessor `buf` (element i at buf(i)).  SRC_LDS: LDG reads the same LDS
+// buffer, so the first stage also waits for every read before writing.  All
+// threads of the block must call this (it synchronizes); threads with
+// act == false only take part in the barriers.
+template <typename T, int SIGN, bool SRC_LDS, int L, class LDG, class STG, class BUF>
+__device__ __forceinline__ void line_fft(LDG ldg, STG stg, BUF buf, const C2<T> *__restrict__ tw, int jb, bool act)
+{
+    line_fft_stage<T, SIGN, SRC_LDS, L, 1>(ldg, stg, buf, tw, jb, act);
+}
+
+template <typename TS, typename TC> struct CLoad;
+template <> struct CLoad<double, double> {
+    __device__ __forceinline__ static C2<double> ld(const double *p, long i)
+    {
+        const double2 x = reinterpret_cast<const double2 *>(p)[i];
+        return {x.x, x.y};
+    }
+    __device__ __forceinline__ static void st(double *p, long i, C2<double> v)
+    {
+        reinterpret_cast<double2 *>(p)[i] = double2{v.x, v.y};
+    }
+};
+template <typename TC> struct CLoadF {
+    __device__ __forceinline__ static C2<TC> ld(const float *p, long i)
+    {
+        const float2 x = reinterpret_cast<const float2 *>(p)[i];
+        return {(TC)x.x, (TC)x.y};
+    }
+    __device__ __forceinline__ static void st(float *p, long i, C2<TC> v)
+    {
+        reinterpret_cast<float2 *>(p)[i] = float2{(float)v.x, (float)v.y};
+    }
+};
+template <> struct CLoad<float, float> : CLoadF<float> {};
+template <> struct CLoad<float, double> : CLoadF<double> {};
+
+// ---------------------------------------------------------------------------
+// passes 2-4: complex FFT of strided lines.  Line (a, b) starts at a + b*sb
+// (complex units), element stride se; a in [0, na) is the contiguous index.
+// Tile: TX = 1 << lgtx consecutive lines; thread = (line, jb), line fastest.
+// MODE 0 forward, 1 inverse, 2 forward -> Poisson multiply -> inverse.
+// ---------------------------------------------------------------------------
+template <typename TS, typename TC, int MODE, int L>
+__global__ void __launch_bounds__(1024) k_fft_strided(TS *__restrict__ data, long na, long sb, int se,
+                                                     const TC *__restrict__ twL, int lgtx, PoissonMul<TC> pm)
+{
+    extern __shared__ unsigned char smem[];
+    C2<TC> *tile = (C2<TC> *)smem;
+    const int TX = 1 << lgtx;
+    const int ln = threadIdx.x & (TX - 1), jb = threadIdx.x >> lgtx;
+    const long a = (long)blockIdx.x * TX + ln;
+    const bool act = a < na;
+    TS *const line0 = data + 2 * ((act ? a : 0) + (long)blockIdx.y * sb);
+    const C2<TC> *tw = (const C2<TC> *)twL;
+    using IO = CLoad<TS, TC>;
+    auto ldg = [&](int i) { return IO::ld(line0, i * se); };
+    auto stg = [&](int i, C2<TC> v) { IO::st(line0, i * se, v); };
+    auto buf = [&](int i) -> C2<TC> & { return tile[(i << lgtx) + ln]; };
+    if constexpr (MODE == 0) {
+        line_fft<TC, -1, false, L>(ldg, stg, buf, tw, jb, act);
+    } else if constexpr (MODE == 1) {
+        line_fft<TC, +1, false, L>(ldg, stg, buf, tw, jb, act);
+    } else {
+        // forward into LDS, scale by -1/|k'|^2, inverse from LDS to HBM
+        const int kx = (int)(a % pm.nxh), ky = (int)(a / pm.nxh);
+        const double kxy = act ? pm.kx2[kx] + pm.ky2[ky] : 0.0;
+        auto sts = [&](int i, C2<TC> v) {
+            const double ks = kxy + pm.kz2[i];
+            double fac;
+            if (pm.zero_policy == 0) fac = (kx == 0 && ky == 0 && i == 0) ? 0.0 : -1.0 / (ks + 1.0e-20);
+            else fac = (ks <= 1.0e-14) ? 0.0 : -1.0 / ks;
+            v.x *= (TC)fac;
+            v.y *= (TC)fac;
+            buf(i) = v;
+        };
+        line_fft<TC, -1, false, L>(ldg, sts, buf, tw, jb, act);
+        __syncthreads();
+        auto ldb = [&](int i) { return buf(i); };
+        line_fft<TC, +1, true, L>(ldb, stg, buf, tw, jb, act);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pass 1: x real-to-complex.  nlines lines of nx reals -> nx/2+1 complex.
+// Tile: TXL lines; thread = (line, jb), jb fastest.
+// ---------------------------------------------------------------------------
+template <typename TS, typename TC, int L>
+__global__ void __launch_bounds__(1024) k_fft_x_r2c(const TS *__restrict__ in, TS *__restrict__ out, long nlines,
+                                                   const TC *__restrict__ twL, const TC *__restrict__ twN, int TXL)
+{
+    extern __shared__ unsigned char smem[];
+    C2<TC> *tile = (C2<TC> *)smem;
+    constexpr int nx = 2 * L, L8 = L >> 3, LDL = L + 1, nxh = L + 1;
+    const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
+    const long l = (long)blockIdx.x * TXL + line;
+    const bool act = (line < TXL) && (l < nlines);
+    using IO = CLoad<TS, TC>;
+    const TS *src = in + (act ? l : 0) * (long)nx;
+    auto ldg = [&](int i) { return IO::ld(src, i); };  // packed z_j = (x_2j, x_2j+1)
+    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + i]; };
+    auto sts = [&](int i, C2<TC> v) { buf(i) = v; };
+    line_fft<TC, -1, false, L>(ldg, sts, buf, (const C2<TC> *)twL, jb, act);
+    __syncthreads();
+    // split: X_k = E_k + W^k O_k, E = (Z_k + conj Z_{L-k})/2, O = (Z_k - conj Z_{L-k})/(2i)
+    const C2<TC> *wN = (const C2<TC> *)twN;
+    TS *dst = out + (act ? l : 0) * (long)nxh * 2;
+    if (act) {
+        for (int k = jb; k < nxh; k += L8) {
+            const C2<TC> zk = buf(k == L ? 0 : k);
+            const C2<TC> zc = buf(k == 0 ? 0 : L - k);
+            const C2<TC> ev = {(TC)0.5 * (zk.x + zc.x), (TC)0.5 * (zk.y - zc.y)};
+            const C2<TC> od = {(TC)0.5 * (zk.y + zc.y), (TC)-0.5 * (zk.x - zc.x)};
+            IO::st(dst, k, cadd(ev, cmul(wN[k], od)));
+        }
+    }
+}
+
+// pass 5: x complex-to-real, scaled by `scale`
+template <typename TS, typename TC, int L>
+__global__ void __launch_bounds__(1024) k_fft_x_c2r(const TS *__restrict__ in, TS *__restrict__ out, long nlines,
+                                                   const TC *__restrict__ twL, const TC *__restrict__ twN, int TXL,
+                                                   TC scale)
+{
+    extern __shared__ unsigned char smem[];
+    C2<TC> *tile = (C2<TC> *)smem;
+    constexpr int nx = 2 * L, L8 = L >> 3, LDL = L + 1, nxh = L + 1;
+    const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
+    const long l = (long)blockIdx.x * TXL + line;
+    const bool act = (line < TXL) && (l < nlines);
+    using IO = CLoad<TS, TC>;
+    const TS *src = in + (act ? l : 0) * (long)nxh * 2;
+    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + i]; };
+    const C2<TC> *wN = (const C2<TC> *)twN;
+    // Z_k = E_k + i O_k, E = (X_k + conj X_{L-k})/2, O = (X_k - conj X_{L-k}) conj(W^k)/2
+    if (act) {
+        for (int k = jb; k < L; k += L8) {
+            const C2<TC> xk = IO::ld(src, k), xc = IO::ld(src, L - k);
+            const C2<TC> ev = {(TC)0.5 * (xk.x + xc.x), (TC)0.5 * (xk.y - xc.y)};
+            const C2<TC> df = {(TC)0.5 * (xk.x - xc.x), (TC)0.5 * (xk.y + xc.y)};
+            const C2<TC> od = cmul(df, C2<TC>{wN[k].x, -wN[k].y});
+            buf(k) = C2<TC>{ev.x - od.y, ev.y + od.x};
+        }
+    }
+    __syncthreads();
+    TS *dst = out + (act ? l : 0) * (long)nx;
+    auto ldb = [&](int i) { return buf(i); };
+    auto stg = [&](int i, C2<TC> v) {
+        v.x *= scale;
+        v.y *= scale;
+        IO::st(dst, i, v);
+    };
+    line_fft<TC, +1, true, L>(ldb, stg, buf, (const C2<TC> *)twL, jb, act);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static int ilog2(int v)
+{
+    int l = 0;
+    while ((1 << l) < v) l++;
+    return l;
+}
+
+template <typename TS, typename TC, int MODE, int L>
+static void launch_strided_fft(TS *spec, long na, long sb, long se, const TC *tw, const PoissonMul<TC> &pm, int nb,
+                               hipStream_t st)
+{
+    const size_t ce = 2 * sizeof(TC);
+    int TX = 1024 / (L / 8);
+    while (TX > 1 && (size_t)TX * L * ce > 64 * 1024) TX >>= 1;
+    int lg = 0;
+    while ((1 << lg) < TX) lg++;
+    dim3 grid((unsigned)((na + TX - 1) / TX), (unsigned)nb);
+    hipLaunchKernelGGL((k_fft_strided<TS, TC, MODE, L>), grid, dim3(TX * (L / 8)), (size_t)TX * L * ce, st, spec, na,
+                       sb, (int)se, tw, lg, pm);
+}
+
+template <typename TS, typename TC, int MODE>
+static void strided_fft(int L, TS *spec, long na, long sb, long se, const TC *tw, const PoissonMul<TC> &pm, int nb,
+                        hipStream_t st)
+{
+    switch (L) {
+    case 8: launch_strided_fft<TS, TC, MODE, 8>(spec, na, sb, se, tw, pm, nb, st); break;
+    case 16: launch_strided_fft<TS, TC, MODE, 16>(spec, na, sb, se, tw, pm, nb, st); break;
+    case 32: launch_strided_fft<TS, TC, MODE, 32>(spec, na, sb, se, tw, pm, nb, st); break;
+    case 64: launch_strided_fft<TS, TC, MODE, 64>(spec, na, sb, se, tw, pm, nb, st); break;
+    case 128: launch_strided_fft<TS, TC, MODE, 128>(spec, na, sb, se, tw, pm, nb, st); break;
+    case 256: launch_strided_fft<TS, TC, MODE, 256>(spec, na, sb, se, tw, pm, nb, st); break;
+    case 512: launch_strided_fft<TS, TC, MODE, 512>(spec, na, sb, se, tw, pm, nb, st); break;
+    case 1024: launch_strided_fft<TS, TC, MODE, 1024>(spec, na, sb, se, tw, pm, nb, st); break;
+    }
+}
+
+template <typename TS, typename TC, int L>
+static void launch_x_fft(bool inverse, const TS *in, TS *out, long nlines, const TC *tw, const TC *twn, TC scale,
+                         hipStream_t st)
+{
+    const size_t ce = 2 * sizeof(TC);
+    int TXL = 1024 / (L / 8);
+    while (TXL > 1 && (size_t)TXL * (L + 1) * ce > 64 * 1024) TXL >>= 1;
+    dim3 grid((unsigned)((nlines + TXL - 1) / TXL));
+    if (!inverse)
+        hipLaunchKernelGGL((k_fft_x_r2c<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * (L + 1) * ce, st, in,
+                           out, nlines, tw, twn, TXL);
+    else
+        hipLaunchKernelGGL((k_fft_x_c2r<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * (L + 1) * ce, st, in,
+                           out, nlines, tw, twn, TXL, scale);
+}
+
+template <typename TS, typename TC>
+static void x_fft(int L, bool inverse, const TS *in, TS *out, long nlines, const TC *tw, const TC *twn, TC scale,
+                  hipStream_t st)
+{
+    switch (L) {
+    case 8: launch_x_fft<TS, TC, 8>(inverse, in, out, nlines, tw, twn, scale, st); break;
+    case 16: launch_x_fft<TS, TC, 16>(inverse, in, out, nlines, tw, twn, scale, st); break;
+    case 32: launch_x_fft<TS, TC, 32>(inverse, in, out, nlines, tw, twn, scale, st); break;
+    case 64: launch_x_fft<TS, TC, 64>(inverse, in, out, nlines, tw, twn, scale, st); break;
+    case 128: launch_x_fft<TS, TC, 128>(inverse, in, out, nlines, tw, twn, scale, st); break;
+    case 256: launch_x_fft<TS, TC, 256>(inverse, in, out, nlines, tw, twn, scale, st); break;
+    case 512: launch_x_fft<TS, TC, 512>(inverse, in, out, nlines, tw, twn, scale, st); break;
+    }
+}
+
+template <typename TS, typename TC>
+static hipError_t run_poisson_t(const PoissonPlanDev &pl, const TS *rhs, TS *f, TS *spec, hipStream_t st)
+{
+    const int nx = pl.nx, ny = pl.ny, nz = pl.nz, nxh = nx / 2 + 1, L = nx / 2;
+    const bool dbl = sizeof(TC) == 8;
+    const TC *twx = (const TC *)(dbl ? (const void *)pl.tw_d[0] : (const void *)pl.tw_f[0]);
+    const TC *twnx = (const TC *)(dbl ? (const void *)pl.twn_d : (const void *)pl.twn_f);
+    const TC *twy = (const TC *)(dbl ? (const void *)pl.tw_d[1] : (const void *)pl.tw_f[1]);
+    const TC *twz = (const TC *)(dbl ? (const void *)pl.tw_d[2] : (const void *)pl.tw_f[2]);
+    PoissonMul<TC> pm{pl.kx2, pl.ky2, pl.kz2, nxh, pl.zero_policy};
+    const long nlines = (long)ny * nz;
+    const TC scale = (TC)(1.0 / ((double)L * ny * nz));
+    x_fft<TS, TC>(L, false, rhs, spec, nlines, twx, twnx, scale, st);
+    strided_fft<TS, TC, 0>(ny, spec, (long)nxh, (long)nxh * ny, (long)nxh, twy, pm, nz, st);
+    strided_fft<TS, TC, 2>(nz, spec, (long)nxh * ny, 0L, (long)nxh * ny, twz, pm, 1, st);
+    strided_fft<TS, TC, 1>(ny, spec, (long)nxh, (long)nxh * ny, (long)nxh, twy, pm, nz, st);
+    x_fft<TS, TC>(L, true, spec, f, nlines, twx, twnx, scale, st);
+    return hipGetLastError();
+}
+
+hipError_t run_poisson(const PoissonPlanDev &pl, int mode, const void *rhs, void *f, void *spec, hipStream_t st)
+{
+    switch (mode) {
+    case 0: return run_poisson_t<double, double>(pl, (const double *)rhs, (double *)f, (double *)spec, st);
+    case 1: return run_poisson_t<float, float>(pl, (const float *)rhs, (float *)f, (float *)spec, st);
+    case 2: return run_poisson_t<float, double>(pl, (const float *)rhs, (float *)f, (float *)spec, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace pdo

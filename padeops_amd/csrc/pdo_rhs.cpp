@@ -1,0 +1,137 @@
+// Full RHS evaluation on a triply periodic n^3 grid: the composite the
+// benchmark times.  Built from the same operator objects the C ABI exposes
+// (cd06 derivatives, cf90 filters, PoissonPeriodic), in the order of the
+// reference call sites: derivatives%ddx/ddy/ddz (derivatives.F90:447-510),
+// filters%filterx/y/z (filters.F90:220-269), PoissonPeriodic%poisson_solve
+// with modified-wavenumber callbacks (PoissonPeriodic.F90:62-87,188-204).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../include/padeops_hip.h"
+#include "pdo_ops.hpp"
+#include "pdo_poisson.hpp"
+
+namespace pdo {
+
+struct RhsObject {
+    int n = 0;
+    double L = 0;
+    int prec = 0;
+    size_t word = 8;
+    OpObject d1, fil;
+    PoissonObject poisson;
+    void *ut[3] = {nullptr, nullptr, nullptr};
+    void *tmp = nullptr, *div = nullptr;
+    double *partial = nullptr;
+    int nblocks = 1024;
+    ~RhsObject()
+    {
+        for (void *p : ut) if (p) (void)hipFree(p);
+        if (tmp) (void)hipFree(tmp);
+        if (div) (void)hipFree(div);
+        if (partial) (void)hipFree(partial);
+    }
+    void *field(void *base, int q) const { return (char *)base + (size_t)q * n * n * n * word; }
+};
+
+}  // namespace pdo
+
+struct pdo_rhs_s : public pdo::RhsObject {};
+
+using namespace pdo;
+
+extern "C" {
+
+int pdo_rhs_init(pdo_rhs *h, int n, double L, int prec)
+{
+    if (!h) return PDO_ERR_ARG;
+    *h = nullptr;
+    if (prec_mode(prec) < 0) { set_error("unknown precision mode"); return PDO_ERR_ARG; }
+    auto *o = new pdo_rhs_s();
+    o->n = n; o->L = L; o->prec = prec; o->word = (prec == PDO_F64) ? 8 : 4;
+    const double dx = L / n;
+    o->d1.scheme = SCHEME_CD06; o->d1.n = n; o->d1.dx = dx; o->d1.periodic = true;
+    o->fil.scheme = SCHEME_CF90; o->fil.n = n; o->fil.dx = 1.0; o->fil.periodic = true;
+    int st = o->poisson.init(n, n, n, dx, dx, dx, PDO_MODK_CD06, PDO_MODK_CD06, PDO_MODK_CD06, 1);
+    if (st) { delete o; return st; }
+    const size_t bytes = (size_t)n * n * n * o->word;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 3 && e == hipSuccess; i++) e = hipMalloc(&o->ut[i], bytes);
+    if (e == hipSuccess) e = hipMalloc(&o->tmp, bytes);
+    if (e == hipSuccess) e = hipMalloc(&o->div, bytes);
+    if (e == hipSuccess) e = hipMalloc(&o->partial, o->nblocks * sizeof(double));
+    if (e != hipSuccess) { delete o; return hip_fail(e, "hipMalloc(rhs work buffers)"); }
+    *h = o;
+    return 0;
+}
+
+int pdo_rhs_eval(pdo_rhs h, const void *u, const void *v, const void *w, void *grad, void *uo, void *vo, void *wo,
+                 void *stream)
+{
+    if (!h || !u || !v || !w || !grad || !uo || !vo || !wo) { set_error("null argument"); return PDO_ERR_ARG; }
+    hipStream_t st = (hipStream_t)stream;
+    const int n = h->n;
+    const long nn = (long)n * n;
+    const void *ui[3] = {u, v, w};
+    void *uoi[3] = {uo, vo, wo};
+    int r;
+    // derivative / filter along direction d of a field (n, n, n)
+    auto op = [&](OpObject *o, int kind, int d, const void *f, void *out, int epi, const void *aux) {
+        return apply(o, kind, 0, 0, d, h->prec, f, out, d == 3 ? nn : n, d == 3 ? 1 : n, st, epi, aux);
+    };
+    // 9 velocity-gradient components
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if ((r = op(&h->d1, KIND_D1, j + 1, ui[i], h->field(grad, 3 * i + j), 0, nullptr))) return r;
+    // filtered velocity ut_i = F_z F_y F_x u_i (uo_i is the x-pass scratch)
+    for (int i = 0; i < 3; i++) {
+        if ((r = op(&h->fil, KIND_FILTER, 1, ui[i], uoi[i], 0, nullptr))) return r;
+        if ((r = op(&h->fil, KIND_FILTER, 2, uoi[i], h->tmp, 0, nullptr))) return r;
+        if ((r = op(&h->fil, KIND_FILTER, 3, h->tmp, h->ut[i], 0, nullptr))) return r;
+    }
+    // divergence of the filtered velocity, accumulated in place
+    if ((r = op(&h->d1, KIND_D1, 1, h->ut[0], h->div, 0, nullptr))) return r;
+    if ((r = op(&h->d1, KIND_D1, 2, h->ut[1], h->div, 1, h->div))) return r;
+    if ((r = op(&h->d1, KIND_D1, 3, h->ut[2], h->div, 1, h->div))) return r;
+    // pressure (in place) and projection u = ut - grad p
+    if ((r = h->poisson.solve(h->prec, h->div, h->div, st))) return r;
+    for (int i = 0; i < 3; i++)
+        if ((r = op(&h->d1, KIND_D1, i + 1, h->div, uoi[i], 2, h->ut[i]))) return r;
+    return 0;
+}
+
+int pdo_rhs_pressure(pdo_rhs h, const void **p)
+{
+    if (!h || !p) return PDO_ERR_ARG;
+    *p = h->div;
+    return 0;
+}
+
+int pdo_rhs_max_divergence(pdo_rhs h, const void *u, const void *v, const void *w, double *maxdiv, void *stream)
+{
+    if (!h || !u || !v || !w || !maxdiv) return PDO_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const int n = h->n;
+    const long nn = (long)n * n;
+    int r;
+    if ((r = apply(&h->d1, KIND_D1, 0, 0, 1, h->prec, u, h->tmp, n, n, st))) return r;
+    if ((r = apply(&h->d1, KIND_D1, 0, 0, 2, h->prec, v, h->tmp, n, n, st, 1, h->tmp))) return r;
+    if ((r = apply(&h->d1, KIND_D1, 0, 0, 3, h->prec, w, h->tmp, nn, 1, st, 1, h->tmp))) return r;
+    hipError_t e = launch_absmax(h->tmp, h->prec == PDO_F64, nn * n, h->partial, h->nblocks, st);
+    if (e != hipSuccess) return hip_fail(e, "absmax");
+    std::vector<double> part(h->nblocks);
+    e = hipMemcpyAsync(part.data(), h->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "divergence readback");
+    *maxdiv = *std::max_element(part.begin(), part.end());
+    return 0;
+}
+
+int pdo_rhs_destroy(pdo_rhs h)
+{
+    delete h;
+    return 0;
+}
+
+}  // extern "C"

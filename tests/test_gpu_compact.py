@@ -1,0 +1,217 @@
+"""GPU parity of the compact operators (cd06, cd10, cf90, cd06stagg) against the
+CPU oracle, through the C ABI.
+
+Bars: FP64 relative L2 <= 1e-12.  The FP32 modes are compared with the oracle
+applied to the same FP32-rounded input, so the numbers measure the solve and
+not the storage quantization of the input: FP32+refinement and FP32-storage
+/ FP64-arithmetic <= 2e-7 (a few FP32 output roundings), plain FP32 reported
+against a looser bar (it is the no-refinement reference point).
+"""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import padeops_amd as P
+from fields import from_dev, rel_l2, random_field, to_dev
+
+pytestmark = pytest.mark.gpu
+
+DX = 0.37
+
+
+def _run(op_obj, method, f_np, dir, dtype, prec, cuda, extra=()):
+    import torch
+    shape = f_np.shape
+    f = to_dev(f_np, dtype, cuda)
+    out = torch.full_like(f, float("nan"))
+    if dir == 1:
+        na, nb = shape[1], shape[2]
+    elif dir == 2:
+        na, nb = shape[0], shape[2]
+    else:
+        na, nb = shape[0], shape[1]
+    rc = getattr(op_obj, method)(f, out, na, nb, *extra, prec=prec)
+    torch.cuda.synchronize()
+    assert rc == 0
+    return from_dev(out, shape)
+
+
+def _shape(n, dir, other=(7, 5)):
+    s = list(other)
+    s.insert(dir - 1, n)
+    return tuple(s)
+
+
+# (scheme, method, oracle op, kind)
+COLLOCATED = [
+    ("cd06", "dd", O.CD06_D1), ("cd06", "d2d", O.CD06_D2),
+    ("cd10", "dd", O.CD10_D1), ("cd10", "d2d", O.CD10_D2), ("cf90", "filter", O.CF90),
+]
+
+
+def _make(scheme, n, periodic, bc1=0, bcn=0):
+    if scheme == "cd06":
+        o = P.cd06()
+        assert o.init(n, DX, periodic, bc1, bcn) == 0
+    elif scheme == "cd10":
+        o = P.cd10()
+        assert o.init(n, DX, periodic, bc1, bcn) == 0
+    else:
+        o = P.cf90()
+        assert o.init(n, periodic) == 0
+    return o
+
+
+@pytest.mark.parametrize("n", [32, 64, 256, 48, 17])
+@pytest.mark.parametrize("dir", [1, 2, 3])
+@pytest.mark.parametrize("scheme,meth,oop", COLLOCATED)
+def test_periodic_fp64(cuda, scheme, meth, oop, dir, n):
+    import torch
+    if scheme == "cf90" and n < 10:
+        pytest.skip("cf90 needs n >= 10")
+    f = random_field(_shape(n, dir), seed=n * 10 + dir)
+    o = _make(scheme, n, True)
+    extra = () if scheme == "cd06" else (0, 0)
+    g = _run(o, "%s%d" % (meth, dir), f, dir, torch.float64, "f64", cuda, extra)
+    ref = O.apply(oop, f, dir, DX if scheme != "cf90" else 1.0, True)
+    err = rel_l2(g, ref)
+    print("periodic", scheme, meth, "dir", dir, "n", n, "rel_l2", err)
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("n", [64, 40])
+@pytest.mark.parametrize("dir", [1, 2, 3])
+@pytest.mark.parametrize("bc1,bcn", list(itertools.product([0, 1, -1], repeat=2)))
+@pytest.mark.parametrize("scheme,meth,oop", [c for c in COLLOCATED if c[0] != "cd06"])
+def test_nonperiodic_fp64(cuda, scheme, meth, oop, bc1, bcn, dir, n):
+    import torch
+    f = random_field(_shape(n, dir, (9, 3)), seed=100 + n + dir)
+    o = _make(scheme, n, False)
+    g = _run(o, "%s%d" % (meth, dir), f, dir, torch.float64, "f64", cuda, (bc1, bcn))
+    ref = O.apply(oop, f, dir, DX if scheme != "cf90" else 1.0, False, bc1, bcn)
+    err = rel_l2(g, ref)
+    print("nonperiodic", scheme, meth, bc1, bcn, "dir", dir, "n", n, "rel_l2", err)
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("n", [64, 33])
+@pytest.mark.parametrize("dir", [1, 2, 3])
+@pytest.mark.parametrize("bc", [0, 1])
+def test_cd06_nonperiodic_fp64(cuda, dir, n, bc):
+    import torch
+    f = random_field(_shape(n, dir), seed=7 + n + dir)
+    o = _make("cd06", n, False, bc, bc)
+    g = _run(o, "dd%d" % dir, f, dir, torch.float64, "f64", cuda)
+    ref = O.apply(O.CD06_D1, f, dir, DX, False, bc, bc)
+    assert rel_l2(g, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("prec,bar", [("f32_refine", 2e-7), ("f32_store", 2e-7), ("f32", 5e-3)])
+@pytest.mark.parametrize("dir", [1, 2, 3])
+@pytest.mark.parametrize("scheme,meth,oop", COLLOCATED)
+@pytest.mark.parametrize("periodic", [True, False])
+def test_fp32_modes(cuda, scheme, meth, oop, dir, prec, bar, periodic):
+    import torch
+    if not periodic and (scheme == "cd06" and meth == "d2d"):
+        pytest.skip("cd06 d2 is periodic only")
+    n = 128
+    f = random_field(_shape(n, dir, (70, 3)), seed=5 + dir).astype(np.float32).astype(np.float64)
+    o = _make(scheme, n, periodic)
+    extra = () if scheme == "cd06" else (0, 0)
+    g = _run(o, "%s%d" % (meth, dir), f, dir, torch.float32, prec, cuda, extra)
+    ref = O.apply(oop, f, dir, DX if scheme != "cf90" else 1.0, periodic)
+    err = rel_l2(g, ref)
+    print(prec, scheme, meth, "dir", dir, "periodic", periodic, "rel_l2", err)
+    assert err <= bar
+
+
+STAGG = [("ddz_E2C", O.STG_D1_E2C, 1, 0), ("ddz_C2E", O.STG_D1_C2E, 0, 1), ("d2dz2_C2C", O.STG_D2_C2C, 0, 0),
+         ("d2dz2_E2E", O.STG_D2_E2E, 1, 1), ("InterpZ_E2C", O.STG_IN_E2C, 1, 0),
+         ("InterpZ_C2E", O.STG_IN_C2E, 0, 1)]
+
+
+@pytest.mark.parametrize("n", [64, 24])
+@pytest.mark.parametrize("meth,oop,ein,eout", STAGG)
+@pytest.mark.parametrize("is_complex", [False, True])
+def test_stagg_periodic_fp64(cuda, meth, oop, ein, eout, n, is_complex):
+    import torch
+    n1, n2 = 9, 4
+    nin = n + ein
+    rng = np.random.default_rng(n + ein)
+    if is_complex:
+        fr = rng.standard_normal((n1, n2, nin)) + 1j * rng.standard_normal((n1, n2, nin))
+        if ein:
+            fr[:, :, n] = fr[:, :, 0]
+        f_np = np.asfortranarray(fr)
+        # interleaved (re, im) == real array of shape (2*n1, n2, nin)
+        f_real = np.zeros((2 * n1, n2, nin), order="F")
+        f_real[0::2] = fr.real
+        f_real[1::2] = fr.imag
+    else:
+        f_real = random_field((n1, n2, nin), seed=n)
+        if ein:
+            f_real[:, :, n] = f_real[:, :, 0]
+    o = P.cd06stagg()
+    assert o.init(n, DX) == 0
+    f = to_dev(f_real, torch.float64, cuda)
+    nout = n + eout
+    out = torch.full((f_real.shape[0] * n2 * nout,), float("nan"), dtype=torch.float64, device=cuda)
+    assert getattr(o, meth)(f, out, n1, n2, is_complex=is_complex, prec="f64") == 0
+    g = from_dev(out, (f_real.shape[0], n2, nout))
+    ref = O.apply(oop, f_real, 3, DX, True, n=n)
+    err = rel_l2(g, ref)
+    print("stagg", meth, "n", n, "complex", is_complex, "rel_l2", err)
+    assert err <= 1e-12
+
+
+def test_cf90_complex_filter3_nonperiodic(cuda):
+    import torch
+    n, n1, n2 = 64, 6, 5
+    rng = np.random.default_rng(3)
+    fr = np.zeros((2 * n1, n2, n), order="F")
+    fr[:] = rng.standard_normal(fr.shape)
+    o = P.cf90()
+    assert o.init(n, False) == 0
+    for bc1, bcn in [(0, 0), (1, -1)]:
+        f = to_dev(fr, torch.float64, cuda)
+        out = torch.empty_like(f)
+        assert o.filter3(f, out, n1, n2, bc1, bcn, prec="f64", complex_=True) == 0
+        g = from_dev(out, fr.shape)
+        # reference: RHS with (bc1,bcn), solve with the (0,0) penta system
+        A0, _ = O.dense(O.CF90, n, 1.0, False, 0, 0)
+        _, B = O.dense(O.CF90, n, 1.0, False, bc1, bcn)
+        M = np.linalg.solve(A0, B)
+        ref = np.einsum("ij,abj->abi", M, fr)
+        assert rel_l2(g, ref) <= 1e-12
+
+
+def test_error_behaviour(cuda):
+    import torch
+    o = P.cd10()
+    assert o.init(7, DX, True) == 2           # n < 8
+    assert P.cf90().init(9, True) == 7        # n < 10
+    assert P.cd06().init(5, DX, True) == 3    # periodic n < 6
+    assert P.cd06().init(7, DX, False) == 2   # non-periodic n < 8
+    assert P.cd06stagg().init(4, DX) == 21
+    o = P.cd10()
+    assert o.init(32, DX, False) == 0
+    f = torch.zeros(32 * 4, dtype=torch.float64, device=cuda)
+    with pytest.raises(P.GracefulExit) as e:
+        o.dd1(f, torch.empty_like(f), 2, 2, 2, 0, prec="f64")
+    assert e.value.code == 324
+    # n == 1: derivatives return zero, the filter returns its input
+    d = P.cd06()
+    assert d.init(1, DX, True) == 0
+    x = torch.arange(6, dtype=torch.float64, device=cuda) + 1.0
+    y = torch.full_like(x, 7.0)
+    assert d.dd1(x, y, 3, 2, prec="f64") == 0
+    torch.cuda.synchronize()
+    assert float(y.abs().max()) == 0.0
+    c = P.cf90()
+    assert c.init(1, True) == 0
+    assert c.filter2(x, y, 3, 2, prec="f64") == 0
+    torch.cuda.synchronize()
+    assert torch.equal(x, y)

@@ -1,0 +1,125 @@
+"""GPU parity of the periodic FFT Poisson solver and of the full RHS
+evaluation against the CPU oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle as O
+import padeops_amd as P
+from fields import fourier_modes, from_dev, isotropic_velocity, random_field, rel_l2, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("shape", [(64, 32, 16), (32, 32, 32), (16, 8, 64), (128, 64, 32)])
+@pytest.mark.parametrize("modk", ["fourier", "cd06", "cd06stagg", "cd10"])
+@pytest.mark.parametrize("zero_policy", [0, 1])
+def test_poisson_fp64(cuda, shape, modk, zero_policy):
+    import torch
+    nx, ny, nz = shape
+    Lx, Ly, Lz = 2 * np.pi, 4 * np.pi, 1.0
+    dx, dy, dz = Lx / nx, Ly / ny, Lz / nz
+    rhs = random_field(shape, seed=sum(shape))
+    if modk == "fourier" or zero_policy == 0:
+        # keep the reference's 1e-20 guard meaningful: a field with no
+        # component on modes where the modified wavenumber vanishes
+        rhs = rhs - rhs.mean()
+    ps = P.PoissonPeriodic()
+    assert ps.init(dx, dy, dz, nx, ny, nz, (modk,) * 3, zero_policy) == 0
+    r = to_dev(rhs, torch.float64, cuda)
+    out = torch.full_like(r, float("nan"))
+    assert ps.poisson_solve(r, out, prec="f64") == 0
+    torch.cuda.synchronize()
+    g = from_dev(out, shape)
+    kind = P.MODK[modk]
+    if zero_policy == 0 and modk in ("cd06", "cd10"):
+        # the 1e-20 guard blows up the all-Nyquist modes for collocated
+        # schemes (k' = 0 there); the reference's own result is then dominated
+        # by round-off: only check finiteness is not meaningful -> skip
+        pytest.skip("reference zero-mode policy is ill-posed for collocated modified wavenumbers")
+    ref = O.poisson_periodic(rhs, dx, dy, dz, kind, zero_policy)
+    err = rel_l2(g, ref)
+    print("poisson", shape, modk, zero_policy, "rel_l2", err)
+    assert err <= 1e-12
+
+
+def test_poisson_analytic(cuda):
+    """tests/test_PoissonPeriodic.F90: 64x32x16, f = sin(l x) cos(m y) sin(n z)."""
+    import torch
+    nx, ny, nz = 64, 32, 16
+    Lx, Ly, Lz = 2 * np.pi, 2 * np.pi, 2 * np.pi
+    x = np.arange(nx) * Lx / nx
+    y = np.arange(ny) * Ly / ny
+    z = np.arange(nz) * Lz / nz
+    X, Y, Zc = np.meshgrid(x, y, z, indexing="ij")
+    l, m, k = 3, 2, 4
+    f = np.sin(l * X) * np.cos(m * Y) * np.sin(k * Zc)
+    rhs = np.asfortranarray(-(l * l + m * m + k * k) * f)
+    ps = P.PoissonPeriodic()
+    assert ps.init(Lx / nx, Ly / ny, Lz / nz, nx, ny, nz) == 0
+    r = to_dev(rhs, torch.float64, cuda)
+    assert ps.poisson_solve(r, prec="f64") == 0   # in place
+    torch.cuda.synchronize()
+    g = from_dev(r, rhs.shape)
+    assert np.abs(g - f).max() < 1e-12
+
+
+@pytest.mark.parametrize("prec,bar", [("f32_refine", 3e-6), ("f32_store", 3e-7)])
+def test_poisson_fp32(cuda, prec, bar):
+    import torch
+    shape = (64, 64, 64)
+    dx = 2 * np.pi / 64
+    rhs = random_field(shape, seed=9).astype(np.float32).astype(np.float64)
+    ps = P.PoissonPeriodic()
+    assert ps.init(dx, dx, dx, *shape, ("cd06",) * 3, 1) == 0
+    r = to_dev(rhs, torch.float32, cuda)
+    out = torch.empty_like(r)
+    assert ps.poisson_solve(r, out, prec=prec) == 0
+    torch.cuda.synchronize()
+    ref = O.poisson_periodic(rhs, dx, dx, dx, 1, 1)
+    err = rel_l2(from_dev(out, shape), ref)
+    print("poisson", prec, "rel_l2", err)
+    assert err <= bar
+
+
+def _rhs_gpu(n, u, v, w, prec, cuda):
+    import torch
+    dt = torch.float64 if prec == "f64" else torch.float32
+    rhs = P.FullRHS(n, 2 * np.pi, prec)
+    U = [to_dev(a, dt, cuda) for a in (u, v, w)]
+    N = n ** 3
+    grad = torch.full((9 * N,), float("nan"), dtype=dt, device=cuda)
+    out = [torch.full((N,), float("nan"), dtype=dt, device=cuda) for _ in range(3)]
+    assert rhs.eval(U[0], U[1], U[2], grad, *out) == 0
+    torch.cuda.synchronize()
+    g = [from_dev(grad[q * N:(q + 1) * N], (n, n, n)) for q in range(9)]
+    o = [from_dev(t, (n, n, n)) for t in out]
+    return rhs, g, o, out
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_full_rhs_fp64(cuda, n):
+    u, v, w = isotropic_velocity(n, k0=4.0, seed=42)
+    rhs, g, o, out_t = _rhs_gpu(n, u, v, w, "f64", cuda)
+    grad_ref, uo, vo, wo, p = O.full_rhs(u, v, w)
+    for q in range(9):
+        e = rel_l2(g[q], grad_ref[q])
+        print("grad", q, e)
+        assert e <= 1e-12
+    for a, b, name in zip(o, (uo, vo, wo), "uvw"):
+        e = rel_l2(a, b)
+        print("projected", name, e)
+        assert e <= 1e-12
+    # divergence of the projected field with the same cd06 operators
+    maxdiv = rhs.max_divergence(*out_t)
+    print("max |div|", maxdiv)
+    assert maxdiv < 1e-12
+
+
+def test_full_rhs_fp32_refine_error(cuda):
+    n = 64
+    u, v, w = isotropic_velocity(n, k0=4.0, seed=7)
+    _, g, o, _ = _rhs_gpu(n, u, v, w, "f32_refine", cuda)
+    grad_ref, uo, vo, wo, _ = O.full_rhs(u, v, w)
+    errs = [rel_l2(g[q], grad_ref[q]) for q in range(9)] + [rel_l2(a, b) for a, b in zip(o, (uo, vo, wo))]
+    print("fp32+refine rel errors", errs)
+    assert max(errs) <= 1e-6
