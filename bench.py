@@ -117,6 +117,17 @@ def main():
     g32, o32 = buffers(torch.float32)
     ms32, wall32 = timed(rhs32, u32, g32, o32)
 
+    # FP32 storage with FP64 arithmetic (no refinement needed), for comparison
+    rhs32s = P.FullRHS(n, L, "f32_store")
+    ms32s, _ = timed(rhs32s, u32, g32, o32)
+    errs_s = {}
+    for q in (0, 4, 8):
+        errs_s["grad%d" % q] = rel_l2(g32[q * N:(q + 1) * N], g64[q * N:(q + 1) * N])
+    for name, a, b in zip(("u_proj", "v_proj", "w_proj"), o32, o64):
+        errs_s[name] = rel_l2(a, b)
+    del rhs32s
+    ms32, wall32 = timed(rhs32, u32, g32, o32)
+
     # accuracy of the FP32+refine evaluation against the FP64 one
     errs = {}
     names = ["dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz"]
@@ -169,6 +180,8 @@ def main():
         "rhs_frac_of_peak_fp64": round(bytes64 / (ms64 * 1e-3) / 1e9 / PEAK_HBM, 4),
         "fp32r_rel_l2_vs_fp64": {k: (None if v is None else float("%.3e" % v)) for k, v in errs.items()},
         "fp64_max_div_after_projection": float("%.3e" % div64),
+        "fp32_store_fp64_arith_evals_per_s": round(1000.0 / ms32s, 3),
+        "fp32_store_rel_l2_vs_fp64": {k: float("%.3e" % v) for k, v in errs_s.items()},
         "roofline": roofline,
         "pass_ms": breakdown,
         "hbm_probe": probe,

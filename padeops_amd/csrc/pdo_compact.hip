@@ -107,6 +107,12 @@ __device__ __forceinline__ T rhs_row(const TW (&win)[W], int j, const T *c, cons
 }
 
 // local banded solve, in place, factors fac[j*5 + {l1,l2,inv,u1,u2}]
+// Scheduling fence every SB rows: the row coefficients are uniform scalar
+// loads; without a fence the scheduler hoists all of them (hundreds of
+// values) and spills scalar registers into vector lanes.
+#define PDO_SCHED_FENCE(j, SB) \
+    if (((j) % (SB)) == (SB) - 1) __builtin_amdgcn_sched_barrier(0)
+
 template <int K, int MM, bool EXACT, typename T>
 __device__ __forceinline__ void local_solve(T (&d)[MM], const T *fac, int m)
 {
@@ -117,6 +123,7 @@ __device__ __forceinline__ void local_solve(T (&d)[MM], const T *fac, int m)
             if (K == 2 && j >= 2) v -= fac[j * 5 + 1] * d[j - 2];
             d[j] = v;
         }
+        PDO_SCHED_FENCE(j, 8);
     }
 #pragma unroll
     for (int j = MM - 1; j >= 0; --j) {
@@ -126,6 +133,7 @@ __device__ __forceinline__ void local_solve(T (&d)[MM], const T *fac, int m)
             if (K == 2 && j + 2 < MM && (EXACT || j + 2 < m)) v -= fac[j * 5 + 4] * d[j + 2];
             d[j] = v;
         }
+        PDO_SCHED_FENCE(j, 8);
     }
 }
 
@@ -140,6 +148,7 @@ __device__ __forceinline__ void apply_spikes(T (&d)[MM], const T *spk, const T (
             for (int q = 0; q < K; ++q) v -= spk[j * 4 + q] * yl[q] + spk[j * 4 + 2 + q] * yr[q];
             d[j] = v;
         }
+        PDO_SCHED_FENCE(j, 8);
     }
 }
 
@@ -155,15 +164,23 @@ __device__ __forceinline__ void apply_spikes(T (&d)[MM], const T *spk, const T (
 // G_0[., d] and per-lane LDS addresses.
 template <int FAM, int MODE, int NPW, bool PER>
 __device__ __forceinline__ void reduce_seg(const typename Mode<MODE>::TC *red, const ApplyArgs &A, int s, int pl,
-                                           typename Mode<MODE>::TC (&yl)[2], typename Mode<MODE>::TC (&yr)[2])
+                                           typename Mode<MODE>::TC (&yl)[2], typename Mode<MODE>::TC (&yr)[2],
+                                           const typename Mode<MODE>::TC *Gtab)
 {
     using TC = typename Mode<MODE>::TC;
     constexpr int K = Fam<FAM>::K;
     const int P = A.P, NR = 2 * K * P;
     yl[0] = yl[1] = yr[0] = yr[1] = TC(0);
     if constexpr (PER) {
-        const TC *G = tab<TC>(A) + A.o_G;
-        for (int dd = 0; dd < P; ++dd) {
+        // only the segment offsets whose coupling is above the working
+        // precision are summed: dd in [-D, D] (all P when 2D+1 >= P)
+        const TC *G = Gtab;
+        const int D = (sizeof(TC) == 8) ? A.D64 : A.D32;
+        const int nd = (2 * D + 1 < P) ? 2 * D + 1 : P;
+        const int d0 = (2 * D + 1 < P) ? P - D : 0;
+        for (int id = 0; id < nd; ++id) {
+            int dd = d0 + id;
+            dd -= (dd >= P) ? P : 0;
             int t = s + dd;
             t -= (t >= P) ? P : 0;
 #pragma unroll
@@ -192,7 +209,7 @@ __device__ __forceinline__ void reduce_seg(const typename Mode<MODE>::TC *red, c
     }
 }
 
-template <int FAM, int MODE, int NPW, bool PER>
+template <int FAM, int MODE, int NPW, bool PER, bool EPI>
 __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
 {
     using TS = typename Mode<MODE>::TS;
@@ -211,8 +228,12 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
     const bool act = a < A.na;
     const int ty = PER ? 1 : (P == 1 ? 3 : (s == 0 ? 0 : (s == P - 1 ? 2 : 1)));
 
-    const TS *__restrict__ fp = (const TS *)A.f + ((act ? a : 0) + (long)blockIdx.y * A.sb_in);
-    const int se = (int)A.se_in;
+    // element offsets fit 32 bits (the host routes larger fields elsewhere):
+    // uniform base + per-lane 32-bit byte offset
+    const char *__restrict__ fb = (const char *)A.f;
+    const unsigned pin = (unsigned)((act ? a : 0) + (long)blockIdx.y * A.sb_in);
+    const unsigned se = (unsigned)A.se_in;
+    auto ldf = [&](unsigned e) { return *(const TS *)(fb + (size_t)(e * (unsigned)sizeof(TS))); };
     TS win[W];
     const int r0 = s * m - R;
 #pragma unroll
@@ -221,7 +242,7 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
         bool ok = true;
         if (PER) r = r < 0 ? r + A.n_sys : (r >= A.n_read ? r - A.n_sys : r);
         else ok = (r >= 0) && (r < A.n_in);
-        win[w] = ok ? fp[r * se] : TS(0);
+        win[w] = ok ? ldf(pin + (unsigned)r * se) : TS(0);
     }
 
     TR cr[5];
@@ -234,10 +255,11 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
         else d[j] = (TC)rhs_row<FAM, TR>(win, j, cr, A, ty, m);
     }
 
-    const TC *fac = tab<TC>(A) + A.o_fac + ty * m * 5;
-    const TC *spk = tab<TC>(A) + A.o_spk + ty * m * 4;
     extern __shared__ unsigned char smem[];
     TC *red = (TC *)smem;
+    const TC *fac = tab<TC>(A) + A.o_fac + ty * m * 5;
+    const TC *spk = tab<TC>(A) + A.o_spk + ty * m * 4;
+    const TC *Gs = tab<TC>(A) + A.o_G;
     TC yl[2], yr[2];
 
     local_solve<K, MM, true>(d, fac, m);
@@ -247,7 +269,7 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
         red[(s * 2 * K + K + q) * NPW + pl] = d[m - K + q];
     }
     __syncthreads();
-    reduce_seg<FAM, MODE, NPW, PER>(red, A, s, pl, yl, yr);
+    reduce_seg<FAM, MODE, NPW, PER>(red, A, s, pl, yl, yr, Gs);
     apply_spikes<K, MM, true>(d, spk, yl, yr, m);
 
     if constexpr (REFINE) {
@@ -272,6 +294,8 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
                 yr[q] = nx >= 0 ? red3[(nx * 2 * K + q) * NPW + pl] : TC(0);
             }
         }
+        // periodic rows all share the interior band (kernel arguments);
+        // non-periodic rows read theirs from the table
         const double *band = A.tab_d + A.o_lhs + ty * m * 5;
         TC e[MM];
 #pragma unroll
@@ -282,7 +306,7 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
                 const int jj = j + o;
                 const double yv = jj < 0 ? (double)yl[K + jj]
                                          : (jj >= m ? (double)yr[jj - m] : (double)d[jj < 0 ? 0 : (jj >= MM ? MM - 1 : jj)]);
-                acc = fma(-band[j * 5 + o + 2], yv, acc);
+                acc = fma(-(PER ? A.band[o + 2] : band[j * 5 + o + 2]), yv, acc);
             }
             e[j] = (TC)acc;
         }
@@ -294,29 +318,48 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
         }
         __syncthreads();
         TC el[2], er[2];
-        reduce_seg<FAM, MODE, NPW, PER>(red2, A, s, pl, el, er);
+        reduce_seg<FAM, MODE, NPW, PER>(red2, A, s, pl, el, er, Gs);
         apply_spikes<K, MM, true>(e, spk, el, er, m);
 #pragma unroll
         for (int j = 0; j < MM; ++j) d[j] += e[j];
     }
 
     if (!act) return;
-    TS *__restrict__ op = (TS *)A.out + (a + (long)blockIdx.y * A.sb_out);
-    const TS *__restrict__ ap = (const TS *)A.aux + (A.aux ? a + (long)blockIdx.y * A.sb_out : 0);
-    const int so = (int)A.se_out;
+    char *ob = (char *)A.out;
+    const char *abase = (const char *)A.aux;
+    const unsigned po = (unsigned)(a + (long)blockIdx.y * A.sb_out);
+    const unsigned so = (unsigned)A.se_out;
+    struct RowPtr {
+        char *b;
+        unsigned p;
+        __device__ TS &operator[](unsigned e) const { return *(TS *)(b + (size_t)((p + e) * (unsigned)sizeof(TS))); }
+    };
+    const RowPtr op{ob, po};
+    const RowPtr ap{(char *)abase, po};
+    if (EPI) {
+        // operand rows are read in chunks before the chunk's output rows are
+        // written (the operand may be the output itself), so each chunk's
+        // loads issue together without holding all 32 rows in registers
+        constexpr int CH = sizeof(TS) == 8 ? 8 : 16;
 #pragma unroll
-    for (int j = 0; j < MM; ++j) {
-        const int idx = (s * m + j) * so;
-        TC v = d[j];
-        if (A.epi == 1) v = (TC)ap[idx] + v;
-        else if (A.epi == 2) v = (TC)ap[idx] - v;
-        op[idx] = (TS)v;
+        for (int j0 = 0; j0 < MM; j0 += CH) {
+            TS av[CH];
+#pragma unroll
+            for (int j = 0; j < CH; ++j) av[j] = ap[(unsigned)(s * m + j0 + j) * so];
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                const TC v = A.epi == 1 ? (TC)av[j] + d[j0 + j] : (TC)av[j] - d[j0 + j];
+                op[(unsigned)(s * m + j0 + j) * so] = (TS)v;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < MM; ++j) op[(unsigned)(s * m + j) * so] = (TS)d[j];
     }
     if (A.n_out > A.n_sys && s == 0) {
-        const int idx = A.n_sys * so;
+        const unsigned idx = (unsigned)A.n_sys * so;
         TC v = d[0];
-        if (A.epi == 1) v = (TC)ap[idx] + v;
-        else if (A.epi == 2) v = (TC)ap[idx] - v;
+        if (EPI) v = A.epi == 1 ? (TC)ap[idx] + v : (TC)ap[idx] - v;
         op[idx] = (TS)v;
     }
 }
@@ -326,7 +369,7 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int padrow(int r) { return r + (r >> 5); }  // one pad element per 32 rows
 
-template <int FAM, int MODE>
+template <int FAM, int MODE, bool EPI>
 __global__ void __launch_bounds__(256) k_xdir(ApplyArgs A, int TP, int LD)
 {
     // periodic pencils only: every segment has the same factors (type 1)
@@ -396,7 +439,12 @@ __global__ void __launch_bounds__(256) k_xdir(ApplyArgs A, int TP, int LD)
         }
         __syncthreads();
         yl[0] = yl[1] = yr[0] = yr[1] = TC(0);
-        for (int dd = 0; dd < P; ++dd) {
+        const int D = (sizeof(TC) == 8) ? A.D64 : A.D32;
+        const int nd = (2 * D + 1 < P) ? 2 * D + 1 : P;
+        const int d0 = (2 * D + 1 < P) ? P - D : 0;
+        for (int id = 0; id < nd; ++id) {
+            int dd = d0 + id;
+            dd -= (dd >= P) ? P : 0;
             // periodic systems are block circulant: G_s[.][t] = G_0[.][t - s]
             const int t = PER ? (s + dd < P ? s + dd : s + dd - P) : dd;
             const TC *G = tab<TC>(A) + A.o_G + (PER ? 0 : (size_t)s * 2 * K * NR);
@@ -420,6 +468,7 @@ __global__ void __launch_bounds__(256) k_xdir(ApplyArgs A, int TP, int LD)
     if constexpr (REFINE) {
         // neighbour rows of the residual from the values the neighbours hold
         TC *rb3 = red + (size_t)2 * TP * P * 2 * K;
+        (void)ty;
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             rb3[(p * P + s) * 2 * K + q] = d[q];
@@ -435,16 +484,15 @@ __global__ void __launch_bounds__(256) k_xdir(ApplyArgs A, int TP, int LD)
                 yr[q] = rb3[(p * P + nx) * 2 * K + q];
             }
         }
-        const double *band = A.tab_d + A.o_lhs + ty * m * 5;
         TC e[MM];
 #pragma unroll
         for (int j = 0; j < MM; ++j) {
-            double acc = rhs_row<FAM, double>(win, j, cr, A, ty, m);
+            double acc = stencil_row<FAM, double>(win, j, cr);
 #pragma unroll
             for (int o = -K; o <= K; ++o) {
                 const int jj = j + o;
                 const double yv = jj < 0 ? (double)yl[K + jj] : (jj >= m ? (double)yr[jj - m] : (double)d[jj < 0 ? 0 : (jj >= MM ? MM - 1 : jj)]);
-                acc = fma(-band[j * 5 + o + 2], yv, acc);
+                acc = fma(-A.band[o + 2], yv, acc);
             }
             e[j] = (TC)acc;
         }
@@ -463,21 +511,33 @@ __global__ void __launch_bounds__(256) k_xdir(ApplyArgs A, int TP, int LD)
     __syncthreads();
     TS *__restrict__ out = (TS *)A.out + pen0 * (long)A.n_out;
     const TS *__restrict__ aux = (const TS *)A.aux + (A.aux ? pen0 * (long)A.n_out : 0);
-#pragma unroll 8
-    for (int v = tid; v < nvec; v += 256) {
+    // 8192 elements per workgroup tile: 32 / VW vectors per thread
+    constexpr int NV = 32 / VW;
+    V ops[NV];
+    if (EPI) {
+        // operand vectors first (the operand may alias the output)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int v = tid + i * 256;
+            if (v < nvec) ops[i] = reinterpret_cast<const V *>(aux)[v];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const int v = tid + i * 256;
+        if (v >= nvec) break;
         const int e = v * VW, pp = e >> lgn, r = e & (A.n_in - 1);
         const TS *src = tile + pp * LD + padrow(r);
         V x;
         TS *xs = reinterpret_cast<TS *>(&x);
 #pragma unroll
-        for (int i = 0; i < VW; ++i) xs[i] = src[i];
-        if (A.epi) {
-            const V y = reinterpret_cast<const V *>(aux)[v];
-            const TS *ys = reinterpret_cast<const TS *>(&y);
+        for (int q = 0; q < VW; ++q) xs[q] = src[q];
+        if (EPI) {
+            const TS *ys = reinterpret_cast<const TS *>(&ops[i]);
 #pragma unroll
-            for (int i = 0; i < VW; ++i) {
-                const TC t = A.epi == 1 ? (TC)ys[i] + (TC)xs[i] : (TC)ys[i] - (TC)xs[i];
-                xs[i] = (TS)t;
+            for (int q = 0; q < VW; ++q) {
+                const TC t = A.epi == 1 ? (TC)ys[q] + (TC)xs[q] : (TC)ys[q] - (TC)xs[q];
+                xs[q] = (TS)t;
             }
         }
         reinterpret_cast<V *>(out)[v] = x;
@@ -592,7 +652,7 @@ hipError_t launch_zero_words(void *p, long words, hipStream_t st)
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int FAM, int MODE, int NPW, bool PER>
+template <int FAM, int MODE, int NPW, bool PER, bool EPI>
 static hipError_t launch_strided_k(const ApplyArgs &A, hipStream_t st)
 {
     using TC = typename Mode<MODE>::TC;
@@ -600,16 +660,30 @@ static hipError_t launch_strided_k(const ApplyArgs &A, hipStream_t st)
     dim3 grid((unsigned)((A.na + NPW - 1) / NPW), (unsigned)A.nb);
     dim3 block(NPW * A.P);
     size_t lds = (size_t)A.P * 2 * K * NPW * sizeof(TC) * (Mode<MODE>::REFINE ? 3 : 1);
-    hipLaunchKernelGGL((k_strided<FAM, MODE, NPW, PER>), grid, block, lds, st, A);
+    hipLaunchKernelGGL((k_strided<FAM, MODE, NPW, PER, EPI>), grid, block, lds, st, A);
     return hipGetLastError();
 }
+
+// epilogue operands (divergence accumulation, projection subtraction) are
+// only used with the periodic cd06 first derivative
+template <int FAM> constexpr bool has_epilogue() { return FAM == 0; }
 
 template <int FAM, int MODE>
 static hipError_t launch_strided_t(const ApplyArgs &A, hipStream_t st)
 {
-    if (!A.periodic) return launch_strided_k<FAM, MODE, 64, false>(A, st);
-    if (A.P % 2 == 0) return launch_strided_k<FAM, MODE, 32, true>(A, st);
-    return launch_strided_k<FAM, MODE, 64, true>(A, st);
+    if (!A.periodic) {
+        if (A.epi) return hipErrorInvalidValue;
+        return launch_strided_k<FAM, MODE, 64, false, false>(A, st);
+    }
+    if (A.epi) {
+        if constexpr (has_epilogue<FAM>()) {
+            if (A.P % 2 == 0) return launch_strided_k<FAM, MODE, 32, true, true>(A, st);
+            return launch_strided_k<FAM, MODE, 64, true, true>(A, st);
+        }
+        return hipErrorInvalidValue;
+    }
+    if (A.P % 2 == 0) return launch_strided_k<FAM, MODE, 32, true, false>(A, st);
+    return launch_strided_k<FAM, MODE, 64, true, false>(A, st);
 }
 
 template <int FAM, int MODE>
@@ -629,7 +703,14 @@ static hipError_t launch_x_t(const ApplyArgs &A, hipStream_t st)
     const long npen = A.na * A.nb;
     dim3 grid((unsigned)((npen + TP - 1) / TP));
     dim3 block(TP * P);
-    hipLaunchKernelGGL((k_xdir<FAM, MODE>), grid, block, lds_of(TP), st, A, TP, LD);
+    if (A.epi) {
+        if constexpr (has_epilogue<FAM>())
+            hipLaunchKernelGGL((k_xdir<FAM, MODE, true>), grid, block, lds_of(TP), st, A, TP, LD);
+        else
+            return hipErrorInvalidValue;
+    } else {
+        hipLaunchKernelGGL((k_xdir<FAM, MODE, false>), grid, block, lds_of(TP), st, A, TP, LD);
+    }
     return hipGetLastError();
 }
 

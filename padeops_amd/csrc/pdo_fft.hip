@@ -234,8 +234,9 @@ __global__ void __launch_bounds__(1024) k_fft_strided(TS *__restrict__ data, lon
     } else if constexpr (MODE == 1) {
         line_fft<TC, +1, false, L>(ldg, stg, buf, tw, jb, act);
     } else {
-        // forward into LDS, scale by -1/|k'|^2, inverse from LDS to HBM
-        const int kx = (int)(a % pm.nxh), ky = (int)(a / pm.nxh);
+        // forward into LDS, scale by -1/|k'|^2, inverse from LDS to HBM;
+        // the line is (kx, ky) = (a, blockIdx.y)
+        const int kx = (int)a, ky = (int)blockIdx.y;
         const double kxy = act ? pm.kx2[kx] + pm.ky2[ky] : 0.0;
         auto sts = [&](int i, C2<TC> v) {
             const double ks = kxy + pm.kz2[i];
@@ -259,7 +260,8 @@ __global__ void __launch_bounds__(1024) k_fft_strided(TS *__restrict__ data, lon
 // ---------------------------------------------------------------------------
 template <typename TS, typename TC, int L>
 __global__ void __launch_bounds__(1024) k_fft_x_r2c(const TS *__restrict__ in, TS *__restrict__ out, long nlines,
-                                                   const TC *__restrict__ twL, const TC *__restrict__ twN, int TXL)
+                                                   const TC *__restrict__ twL, const TC *__restrict__ twN, int TXL,
+                                                   int nxp)
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
@@ -276,7 +278,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c(const TS *__restrict__ in, T
     __syncthreads();
     // split: X_k = E_k + W^k O_k, E = (Z_k + conj Z_{L-k})/2, O = (Z_k - conj Z_{L-k})/(2i)
     const C2<TC> *wN = (const C2<TC> *)twN;
-    TS *dst = out + (act ? l : 0) * (long)nxh * 2;
+    TS *dst = out + (act ? l : 0) * (long)nxp * 2;
     if (act) {
         for (int k = jb; k < nxh; k += L8) {
             const C2<TC> zk = buf(k == L ? 0 : k);
@@ -292,7 +294,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c(const TS *__restrict__ in, T
 template <typename TS, typename TC, int L>
 __global__ void __launch_bounds__(1024) k_fft_x_c2r(const TS *__restrict__ in, TS *__restrict__ out, long nlines,
                                                    const TC *__restrict__ twL, const TC *__restrict__ twN, int TXL,
-                                                   TC scale)
+                                                   TC scale, int nxp)
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
@@ -301,17 +303,29 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r(const TS *__restrict__ in, T
     const long l = (long)blockIdx.x * TXL + line;
     const bool act = (line < TXL) && (l < nlines);
     using IO = CLoad<TS, TC>;
-    const TS *src = in + (act ? l : 0) * (long)nxh * 2;
+    const TS *src = in + (act ? l : 0) * (long)nxp * 2;
     auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + i]; };
     const C2<TC> *wN = (const C2<TC> *)twN;
-    // Z_k = E_k + i O_k, E = (X_k + conj X_{L-k})/2, O = (X_k - conj X_{L-k}) conj(W^k)/2
+    // stage the nx/2+1 modes of the line in LDS (coalesced), then form
+    // Z_k = E_k + i O_k in place, one thread per pair (k, L-k):
+    // E = (X_k + conj X_{L-k})/2, O = (X_k - conj X_{L-k}) conj(W^k)/2
+    if (act)
+        for (int k = jb; k < nxh; k += L8) buf(k) = IO::ld(src, k);
+    __syncthreads();
     if (act) {
-        for (int k = jb; k < L; k += L8) {
-            const C2<TC> xk = IO::ld(src, k), xc = IO::ld(src, L - k);
-            const C2<TC> ev = {(TC)0.5 * (xk.x + xc.x), (TC)0.5 * (xk.y - xc.y)};
-            const C2<TC> df = {(TC)0.5 * (xk.x - xc.x), (TC)0.5 * (xk.y + xc.y)};
-            const C2<TC> od = cmul(df, C2<TC>{wN[k].x, -wN[k].y});
-            buf(k) = C2<TC>{ev.x - od.y, ev.y + od.x};
+        for (int k = jb; k <= L / 2; k += L8) {
+            const int kc = L - k;
+            const C2<TC> xk = buf(k), xc = buf(kc);
+            auto z = [&](C2<TC> a, C2<TC> b, int kk) {
+                const C2<TC> ev = {(TC)0.5 * (a.x + b.x), (TC)0.5 * (a.y - b.y)};
+                const C2<TC> df = {(TC)0.5 * (a.x - b.x), (TC)0.5 * (a.y + b.y)};
+                const C2<TC> od = cmul(df, C2<TC>{wN[kk].x, -wN[kk].y});
+                return C2<TC>{ev.x - od.y, ev.y + od.x};
+            };
+            const C2<TC> zk = z(xk, xc, k);
+            const C2<TC> zc = z(xc, xk, kc);
+            buf(k) = zk;
+            if (k > 0 && kc != k) buf(kc) = zc;
         }
     }
     __syncthreads();
@@ -367,7 +381,7 @@ static void strided_fft(int L, TS *spec, long na, long sb, long se, const TC *tw
 
 template <typename TS, typename TC, int L>
 static void launch_x_fft(bool inverse, const TS *in, TS *out, long nlines, const TC *tw, const TC *twn, TC scale,
-                         hipStream_t st)
+                         int nxp, hipStream_t st)
 {
     const size_t ce = 2 * sizeof(TC);
     int TXL = 1024 / (L / 8);
@@ -375,24 +389,24 @@ static void launch_x_fft(bool inverse, const TS *in, TS *out, long nlines, const
     dim3 grid((unsigned)((nlines + TXL - 1) / TXL));
     if (!inverse)
         hipLaunchKernelGGL((k_fft_x_r2c<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * (L + 1) * ce, st, in,
-                           out, nlines, tw, twn, TXL);
+                           out, nlines, tw, twn, TXL, nxp);
     else
         hipLaunchKernelGGL((k_fft_x_c2r<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * (L + 1) * ce, st, in,
-                           out, nlines, tw, twn, TXL, scale);
+                           out, nlines, tw, twn, TXL, scale, nxp);
 }
 
 template <typename TS, typename TC>
 static void x_fft(int L, bool inverse, const TS *in, TS *out, long nlines, const TC *tw, const TC *twn, TC scale,
-                  hipStream_t st)
+                  int nxp, hipStream_t st)
 {
     switch (L) {
-    case 8: launch_x_fft<TS, TC, 8>(inverse, in, out, nlines, tw, twn, scale, st); break;
-    case 16: launch_x_fft<TS, TC, 16>(inverse, in, out, nlines, tw, twn, scale, st); break;
-    case 32: launch_x_fft<TS, TC, 32>(inverse, in, out, nlines, tw, twn, scale, st); break;
-    case 64: launch_x_fft<TS, TC, 64>(inverse, in, out, nlines, tw, twn, scale, st); break;
-    case 128: launch_x_fft<TS, TC, 128>(inverse, in, out, nlines, tw, twn, scale, st); break;
-    case 256: launch_x_fft<TS, TC, 256>(inverse, in, out, nlines, tw, twn, scale, st); break;
-    case 512: launch_x_fft<TS, TC, 512>(inverse, in, out, nlines, tw, twn, scale, st); break;
+    case 8: launch_x_fft<TS, TC, 8>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
+    case 16: launch_x_fft<TS, TC, 16>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
+    case 32: launch_x_fft<TS, TC, 32>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
+    case 64: launch_x_fft<TS, TC, 64>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
+    case 128: launch_x_fft<TS, TC, 128>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
+    case 256: launch_x_fft<TS, TC, 256>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
+    case 512: launch_x_fft<TS, TC, 512>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
     }
 }
 
@@ -408,11 +422,14 @@ static hipError_t run_poisson_t(const PoissonPlanDev &pl, const TS *rhs, TS *f, 
     PoissonMul<TC> pm{pl.kx2, pl.ky2, pl.kz2, nxh, pl.zero_policy};
     const long nlines = (long)ny * nz;
     const TC scale = (TC)(1.0 / ((double)L * ny * nz));
-    x_fft<TS, TC>(L, false, rhs, spec, nlines, twx, twnx, scale, st);
-    strided_fft<TS, TC, 0>(ny, spec, (long)nxh, (long)nxh * ny, (long)nxh, twy, pm, nz, st);
-    strided_fft<TS, TC, 2>(nz, spec, (long)nxh * ny, 0L, (long)nxh * ny, twz, pm, 1, st);
-    strided_fft<TS, TC, 1>(ny, spec, (long)nxh, (long)nxh * ny, (long)nxh, twy, pm, nz, st);
-    x_fft<TS, TC>(L, true, spec, f, nlines, twx, twnx, scale, st);
+    // half spectrum (nxh, ny, nz) stored with row pitch nxp (a multiple of 16
+    // complex values) so every tile row starts on a 128-byte boundary
+    const int nxp = pl.nxp;
+    x_fft<TS, TC>(L, false, rhs, spec, nlines, twx, twnx, scale, nxp, st);
+    strided_fft<TS, TC, 0>(ny, spec, (long)nxh, (long)nxp * ny, (long)nxp, twy, pm, nz, st);
+    strided_fft<TS, TC, 2>(nz, spec, (long)nxh, (long)nxp, (long)nxp * ny, twz, pm, ny, st);
+    strided_fft<TS, TC, 1>(ny, spec, (long)nxh, (long)nxp * ny, (long)nxp, twy, pm, nz, st);
+    x_fft<TS, TC>(L, true, spec, f, nlines, twx, twnx, scale, nxp, st);
     return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@ template <typename TC> struct PoissonMul {
 
 struct PoissonPlanDev {
     int nx, ny, nz;
+    int nxp;                             // row pitch of the half spectrum (>= nx/2+1, multiple of 16)
     int zero_policy;
     const double *kx2, *ky2, *kz2;       // device, sizes nx/2+1, ny, nz
     const double *tw_d[3];               // exp(-2 pi i m / L) for x (L = nx/2), y, z
@@ -23,7 +24,7 @@ struct PoissonPlanDev {
 };
 
 // mode 0: FP64, 1: FP32 storage and arithmetic, 2: FP32 storage, FP64 arithmetic.
-// spec: work buffer of (nx/2+1)*ny*nz complex values of the storage type.
+// spec: work buffer of nxp*ny*nz complex values of the storage type.
 hipError_t run_poisson(const PoissonPlanDev &pl, int mode, const void *rhs, void *f, void *spec, hipStream_t st);
 
 }  // namespace pdo

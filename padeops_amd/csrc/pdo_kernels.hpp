@@ -27,11 +27,13 @@ struct ApplyArgs {
     int n_sys, n_in, n_read, n_out;
     int m, P;
     int lgn;           // log2(n_in) (x kernel; n_in is a power of two there)
+    int D32, D64;      // circulant coupling offsets kept by FP32 / FP64 solves
     int nbl, nbr, wb;
     const double *tab_d;
     const float *tab_f;
     int o_fac, o_spk, o_lhs, o_G, o_sp;
     double c[5];
+    double band[5];    // interior LHS band (offsets -2..2), used by periodic systems
 };
 
 hipError_t launch_absmax(const void *x, bool is_double, long n, double *partial, int nblocks, hipStream_t st);

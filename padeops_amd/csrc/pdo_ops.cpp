@@ -154,11 +154,13 @@ int apply(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, const voi
     A.n_sys = s.n_sys; A.n_in = s.n_in; A.n_out = s.n_out;
     A.n_read = (s.family == FAM_STG_E2C_A || s.family == FAM_STG_E2C_S) ? s.n_sys + 1 : s.n_sys;
     A.m = t.m; A.P = t.P; A.nbl = t.nbl; A.nbr = t.nbr; A.wb = t.wb;
+    A.D32 = t.D32; A.D64 = t.D64;
     A.lgn = 0;
     while ((1 << A.lgn) < s.n_in) A.lgn++;
     A.tab_d = ts->d_tab; A.tab_f = ts->f_tab;
     A.o_fac = ts->o_fac; A.o_spk = ts->o_spk; A.o_lhs = ts->o_lhs; A.o_G = ts->o_G; A.o_sp = ts->o_sp;
     for (int q = 0; q < 5; q++) A.c[q] = s.c[q];
+    for (int q = 0; q < 5; q++) A.band[q] = s.lhs[s.n_sys / 2][q];
     A.na = A_na; A.nb = A_nb;
     if (dir == 1) {
         // f(n, na, nb): pencil (a,b) at (a + na*b)*n

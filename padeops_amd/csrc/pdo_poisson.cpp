@@ -93,6 +93,7 @@ int PoissonObject::init(int nx_, int ny_, int nz_, double dx, double dy, double 
     std::memset(&dev, 0, sizeof(dev));
     dev.nx = nx; dev.ny = ny; dev.nz = nz; dev.zero_policy = zp;
     const int nxh = nx / 2 + 1;
+    dev.nxp = (nxh + 15) / 16 * 16;
     auto k2 = [&](int n, double d, int kind, int count) {
         std::vector<double> k = wavenums(n, d), out(count);
         for (int i = 0; i < count; i++) {
@@ -145,7 +146,7 @@ int PoissonObject::solve(int prec, const void *rhs, void *f, hipStream_t st)
     case PDO_F32_STORE: mode = 2; break;
     default: set_error("unknown precision mode"); return PDO_ERR_ARG;
     }
-    const size_t need = (size_t)(nx / 2 + 1) * ny * nz * 2 * (mode == 0 ? 8 : 4);
+    const size_t need = (size_t)dev.nxp * ny * nz * 2 * (mode == 0 ? 8 : 4);
     if (spec_bytes < need) {
         if (spec) (void)hipFree(spec);
         spec = nullptr; spec_bytes = 0;

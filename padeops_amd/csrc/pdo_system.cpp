@@ -573,6 +573,24 @@ int build_spike(const SystemDesc &s, SpikeTables &t, bool force_general)
         }
     }
     invert(R, NR);
+    // decay of the (block circulant, periodic) coupling with segment offset
+    t.D32 = t.D64 = P;
+    if (s.periodic && P > 1) {
+        std::vector<double> blk(P, 0.0);
+        for (int q = 0; q < 2 * K; q++)
+            for (int dd = 0; dd < P; dd++)
+                for (int r = 0; r < 2 * K; r++)
+                    blk[dd] = std::max(blk[dd], std::fabs(R[((P - 1) * 2 * K + q) * NR + ((dd + P - 1) % P) * 2 * K + r]));
+        const double top = *std::max_element(blk.begin(), blk.end());
+        auto reach = [&](double tol) {
+            int D = 0;
+            for (int dd = 1; dd <= P / 2; dd++)
+                if (blk[dd] > tol * top || blk[P - dd] > tol * top) D = dd;
+            return D + 1;  // G rows of segment s are R rows of its neighbours (offset +-1)
+        };
+        t.D32 = reach(1e-9);
+        t.D64 = reach(1e-18);
+    }
     for (int sg = 0; sg < P; sg++) {
         const int pv = prev_of(sg), nx = next_of(sg);
         for (int r = 0; r < K; r++) {

@@ -84,6 +84,7 @@ struct SpikeTables {
     std::vector<double> lhs;   // [4][m][5]: band of A (offsets -2..2)
     std::vector<double> G;     // [P][2K][2K*P]: rows of R^-1 giving yL[0..K), yR[0..K)
     std::vector<double> sp;    // [2][NBMAX][WBMAX]: closure rows (left: cols 0..WBMAX-1, right: cols n_in-WBMAX..n_in-1)
+    int D32 = 0, D64 = 0;      // periodic: segment offsets +-D whose coupling exceeds 1e-9 / 1e-18 of the largest
 };
 
 int build_spike(const SystemDesc &s, SpikeTables &t, bool force_general = false);
