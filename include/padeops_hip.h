@@ -62,6 +62,8 @@ enum pdo_modk {
 typedef struct pdo_op_s *pdo_op;
 typedef struct pdo_poisson_s *pdo_poisson;
 typedef struct pdo_rhs_s *pdo_rhs;
+typedef struct pdo_spectral_s *pdo_spectral;
+typedef struct pdo_padepoisson_s *pdo_padepoisson;
 
 /* ---- library ------------------------------------------------------------ */
 const char *pdo_last_error(void);
@@ -124,6 +126,11 @@ int pdo_cf90_filter3_cmplx(pdo_op h, int prec, const void *f, void *fil, int na,
  *   ddz_E2C :820-845  ddz_C2E :846-873  d2dz2_C2C :1008-1026
  *   d2dz2_E2E :1027-1044  InterpZ_E2C :932-957  InterpZ_C2E :958-985 */
 int pdo_cd06stagg_init_periodic(pdo_op *h, int n, double dz);
+/* non-periodic init (cd06stagg.F90:197-232): even / odd reflection at each
+ * wall, or the one-sided closures where is*Sided != 0 (first derivatives and
+ * interpolation; D2 and ddz_E2E always use the even / odd flags).  Returns 21
+ * for n <= 4 like the reference; n in 5..7 is not supported here. */
+int pdo_cd06stagg_init(pdo_op *h, int n, double dz, int isTopEven, int isBotEven, int isTopSided, int isBotSided);
 int pdo_cd06stagg_ddz_E2C(pdo_op h, int prec, const void *fE, void *dfC, int n1, int n2, int is_complex, void *stream);
 int pdo_cd06stagg_ddz_C2E(pdo_op h, int prec, const void *fC, void *dfE, int n1, int n2, int is_complex, void *stream);
 int pdo_cd06stagg_d2dz2_C2C(pdo_op h, int prec, const void *fC, void *d2fC, int n1, int n2, int is_complex,
@@ -134,6 +141,10 @@ int pdo_cd06stagg_interpz_E2C(pdo_op h, int prec, const void *fE, void *ifC, int
                               void *stream);
 int pdo_cd06stagg_interpz_C2E(pdo_op h, int prec, const void *fC, void *ifE, int n1, int n2, int is_complex,
                               void *stream);
+/* collocated first derivatives of the non-periodic object: ddz_C2C :883-903,
+ * ddz_E2E :905-925 (PDO_ERR_UNSUP on a periodic object) */
+int pdo_cd06stagg_ddz_C2C(pdo_op h, int prec, const void *fC, void *dfC, int n1, int n2, int is_complex, void *stream);
+int pdo_cd06stagg_ddz_E2E(pdo_op h, int prec, const void *fE, void *dfE, int n1, int n2, int is_complex, void *stream);
 
 int pdo_op_destroy(pdo_op h);
 
@@ -141,7 +152,8 @@ int pdo_op_destroy(pdo_op h);
  * system A y = B f it solves, A n x n and B n x n_in, row-major FP64.  kind:
  * 0 first derivative, 1 second derivative, 2 filter, 10.. staggered
  * (10 ddz_E2C, 11 ddz_C2E, 12 d2dz2_C2C, 13 d2dz2_E2E, 14 InterpZ_E2C,
- * 15 InterpZ_C2E).  Used to check the tables against the oracle. */
+ * 15 InterpZ_C2E, 16 ddz_C2C, 17 ddz_E2E).  Used to check the tables against
+ * the oracle. */
 int pdo_op_dense(pdo_op h, int kind, int bc1, int bcn, double *A, double *B);
 /* Size query for pdo_op_dense: *n_sys rows, *n_in columns of B. */
 int pdo_op_dims(pdo_op h, int kind, int *n_sys, int *n_in);
@@ -169,6 +181,54 @@ int pdo_poisson_periodic_init(pdo_poisson *h, int nx, int ny, int nz, double dx,
                               int modk_y, int modk_z, int zero_policy);
 int pdo_poisson_periodic_solve(pdo_poisson h, int prec, const void *rhs, void *f, void *stream);
 int pdo_poisson_destroy(pdo_poisson h);
+
+/* ---- spectral: 2D transforms of the igrid solvers --------------------------
+ * replaces type(spectral) of src/incompressible/spectral.F90 as igrid uses it
+ * (dimTransform = 2, x input pencil, igrid.F90:489-493):
+ *   init            :867-931, initializeEverything :933-1260 (TwoPeriodic),
+ *                   init_periodic_inZ_procedures :755-865 (periodic_z != 0)
+ *   fft / ifft      :1413-1453 -> fft_3d.F90:645-663 / :616-643: x r2c then y
+ *                   c2c per z plane, unnormalized forward, inverse x 1/(nx ny);
+ *                   set_oddball drops the kx = nx/2 modes before the c2r
+ *   mTimes_ik1/2    :235-312 (out = i k f, in == out allowed)
+ *   dealias         :314-341: without periodic z the 2/3 rule on (k1, k2)
+ *                   (hard-coded 2/3, :1135-1148); with periodic z a z
+ *                   transform, the dealias_fact mask on (k1, k2, k3), and the
+ *                   inverse times 1/nz
+ *   dealias_edgeField :343-363 (nz+1 planes, periodic z only)
+ * Real fields are (nx, ny, nz), spectra (nx/2+1, ny, nz) interleaved complex,
+ * both unpadded.  Wavenumbers carry the oddball with a + sign, and zero when
+ * fix_oddball != 0 (igrid passes 0).  nx a power of two in [16, 1024], ny in
+ * [8, 1024]; with periodic_z, nz a power of two in [8, 1024]. */
+int pdo_spectral_init(pdo_spectral *h, int nx, int ny, int nz, double dx, double dy, double dz, int fix_oddball,
+                      double dealias_fact, int periodic_z);
+int pdo_spectral_fft(pdo_spectral h, int prec, const void *in, void *out, void *stream);
+int pdo_spectral_ifft(pdo_spectral h, int prec, const void *in, void *out, int set_oddball, void *stream);
+int pdo_spectral_mtimes_ik(pdo_spectral h, int prec, int dir, const void *in, void *out, void *stream);
+int pdo_spectral_dealias(pdo_spectral h, int prec, void *fhat, void *stream);
+int pdo_spectral_dealias_edge(pdo_spectral h, int prec, void *fhat, void *stream);
+/* host copies of k1 (nx/2+1), k2 (ny), k3 (nz) and the dealias factors g1, g2,
+ * g3 (0 or 1, the mask is their product); NULL skips an array */
+int pdo_spectral_wavenumbers(pdo_spectral h, double *k1, double *k2, double *k3, double *g1, double *g2,
+                             double *g3);
+int pdo_spectral_destroy(pdo_spectral h);
+
+/* ---- PadePoisson: periodic pressure projection of igrid ---------------------
+ * replaces the periodic-in-z branch of type(padepoisson),
+ * src/incompressible/PadePoisson.F90, with derivZ = Pade6stagg(scheme cd06,
+ * periodic):
+ *   init (PeriodicInZ) :130-180 + InitPeriodicPoissonSolver :76-128:
+ *       |k|^2 = k1^2 + k2^2 + k3'^2, k3' the cd06stagg modified wavenumber,
+ *       modes with |k|^2 <= 1e-14 get no pressure
+ *   PressureProjection -> PeriodicProjection :386-432 on (uhat, vhat) of
+ *       (nx/2+1, ny, nz) and what of (nx/2+1, ny, nz+1), in place
+ *   DivergenceCheck :1165-1244 (steps 1-3): the real divergence field
+ *       (nx, ny, nz) of the spectral velocity. */
+int pdo_padepoisson_init_periodic(pdo_padepoisson *h, pdo_spectral sp, double dz);
+int pdo_padepoisson_projection(pdo_padepoisson h, int prec, void *uhat, void *vhat, void *what, void *stream);
+int pdo_padepoisson_divergence(pdo_padepoisson h, int prec, const void *uhat, const void *vhat, const void *what,
+                               void *divergence, void *stream);
+int pdo_padepoisson_destroy(pdo_padepoisson h);
 
 /* ---- full RHS evaluation (the benchmarked composite) ----------------------
  * On a triply periodic n^3 grid of side L, for velocity (u, v, w):

@@ -15,6 +15,9 @@ import numpy as np
 
 CD06_D1, CD06_D2, CD10_D1, CD10_D2, CF90 = 1, 2, 3, 4, 5
 STG_D1_E2C, STG_D1_C2E, STG_D2_C2C, STG_D2_E2E, STG_IN_E2C, STG_IN_C2E = 10, 11, 12, 13, 14, 15
+STG_D1_C2C, STG_D1_E2E = 16, 17
+# flag bits of the non-periodic staggered object (cd06stagg%init_nonperiodic)
+TOP_EVEN, BOT_EVEN, TOP_SIDED, BOT_SIDED = 1, 2, 4, 8
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
@@ -42,6 +45,9 @@ def lib():
         L.orc_modk.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
         L.orc_full_rhs.argtypes = [ctypes.c_int, ctypes.c_double] + [dp] * 8
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_stagg_np.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, dp, dp]
+        L.orc_stagg_np_dense.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -142,3 +148,32 @@ def full_rhs(u, v, w, L=2 * np.pi):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def stagg_lengths(op, n):
+    """(n_in, n_out) of a non-periodic staggered operator on n cells."""
+    nin = n + 1 if op in (STG_D1_E2C, STG_IN_E2C, STG_D1_E2E, STG_D2_E2E) else n
+    nout = n + 1 if op in (STG_D1_C2E, STG_IN_C2E, STG_D1_E2E, STG_D2_E2E) else n
+    return nin, nout
+
+
+def stagg_np(op, f, dz, flags, n):
+    """Non-periodic cd06stagg operator along z of f(n1, n2, n_in), flags as TOP_EVEN | ..."""
+    f = np.asfortranarray(f, dtype=np.float64)
+    n1, n2, nin = f.shape
+    assert nin == stagg_lengths(op, n)[0]
+    out = np.zeros((n1, n2, stagg_lengths(op, n)[1]), order="F")
+    ierr = lib().orc_stagg_np(op, n, dz, flags, n1, n2, _p(f), _p(out))
+    if ierr:
+        raise OracleError(ierr)
+    return out
+
+
+def stagg_np_dense(op, n, dz, flags):
+    nin, nout = stagg_lengths(op, n)
+    A = np.zeros((nout, nout))
+    B = np.zeros((nout, nin))
+    ierr = lib().orc_stagg_np_dense(op, n, dz, flags, _p(A), _p(B))
+    if ierr:
+        raise OracleError(ierr)
+    return A, B

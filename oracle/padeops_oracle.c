@@ -970,3 +970,346 @@ int orc_num_threads(void)
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------------ */
+/* cd06stagg, non-periodic (cd06stagg.F90:197-232 + STAGG_CD06_files)        */
+/* ------------------------------------------------------------------------ */
+/* Extra operator codes of the staggered object (collocated first
+ * derivatives on the cell / edge grids, cd06stagg.F90:883-925). */
+enum { ORC_STG_D1_C2C = 16, ORC_STG_D1_E2E = 17 };
+
+/* Lele closure constants of cd06stagg.F90:18-57 */
+#define SG_ALPHA 3.0
+#define SG_P (17.0 / 6.0)
+#define SG_Q (3.0 / 2.0)
+#define SG_R (3.0 / 2.0)
+#define SG_S (-1.0 / 6.0)
+
+static void stg_lele(double *w1, double *w2, double *w3, double *alpha_p, double *q_p, double *alpha_pp,
+                     double *q_pp, double *r_pp)
+{
+    const double ah = 1.0 / 3.0;
+    const double q = SG_Q, r = SG_R, s = SG_S;
+    *q_p = 3.0 / 4.0;
+    *alpha_p = 1.0 / 4.0;
+    *alpha_pp = ((40 * ah - 1) * q + 7 * (4 * ah - 1) * s) / (16 * (ah + 2) * q + 8 * (1 - 4 * ah) * s);
+    *q_pp = (1.0 / 3.0) * (*alpha_pp + 2);
+    *r_pp = (1.0 / 12.0) * (4 * (*alpha_pp) - 1);
+    *w1 = (2 * ah + 1) / (2 * (q + s));
+    *w2 = ((8 * ah + 7) * q - 6 * (2 * ah + 1) * r + (8 * ah + 7) * s) / (9 * (q + s));
+    *w3 = (4 * (ah + 2) * q + 2 * (1 - 4 * ah) * s) / (9 * (q + s));
+}
+
+/* Tridiagonal rows (ddn, dg, dup), 1-based, of ComputeTri* in
+ * STAGG_CD06_files/ComputeTri_allRoutines.F90. */
+static int stg_tri(int op, int n, int te, int be, int ts, int bs, double *ddn, double *dg, double *dup)
+{
+    const double w0s = 223.0 / 186.0, w1s = 61.0 / 62.0;
+    double w1, w2, w3, alpha_p, q_p, alpha_pp, q_pp, r_pp;
+    stg_lele(&w1, &w2, &w3, &alpha_p, &q_p, &alpha_pp, &q_pp, &r_pp);
+    int N;
+    double al;
+    switch (op) {
+    case ORC_STG_D1_E2C: N = n; al = 9.0 / 62.0; break;
+    case ORC_STG_D1_C2E: N = n + 1; al = 9.0 / 62.0; break;
+    case ORC_STG_D1_C2C: N = n; al = 1.0 / 3.0; break;
+    case ORC_STG_D1_E2E: N = n + 1; al = 1.0 / 3.0; break;
+    case ORC_STG_IN_C2E: N = n + 1; al = 3.0 / 10.0; break;
+    case ORC_STG_IN_E2C: N = n; al = 3.0 / 10.0; break;
+    case ORC_STG_D2_E2E: N = n + 1; al = 2.0 / 11.0; break;
+    case ORC_STG_D2_C2C: N = n; al = 2.0 / 11.0; break;
+    default: return -1;
+    }
+    for (int i = 1; i <= N; i++) { ddn[i] = al; dg[i] = 1.0; dup[i] = al; }
+    switch (op) {
+    case ORC_STG_D1_E2C: {  /* ComputeTriD1_E2C :1-50 */
+        const double alpha1 = 37.0 / 183.0, alpha0 = -1.0;
+        if (ts) {
+            ddn[N] = w0s * alpha0; dg[N] = w0s;
+            ddn[N - 1] = w1s * alpha1; dup[N - 1] = w1s * alpha1; dg[N - 1] = w1s;
+        } else if (te) { dup[N] = 0; dg[N] = 1 - al; }
+        else { dg[N] = 1 + al; dup[N] = 0; }
+        if (bs) {
+            dup[1] = w0s * alpha0; dg[1] = w0s;
+            ddn[2] = w1s * alpha1; dup[2] = w1s * alpha1; dg[2] = w1s;
+        } else if (be) { ddn[1] = 0; dg[1] = 1 - al; }
+        else { ddn[1] = 0; dg[1] = 1 + al; }
+        break;
+    }
+    case ORC_STG_D1_C2E:  /* ComputeTriD1_C2E :52-100 */
+        if (ts) { ddn[N] = 0; dup[N] = 0; ddn[N - 1] = 1.0 / 22.0; dup[N - 1] = 1.0 / 22.0; }
+        else if (te) { dup[N] = 0; ddn[N] = 0; }
+        else { ddn[N] = 2 * al; dup[N] = 0; }
+        if (bs) { dup[1] = 0; ddn[1] = 0; dup[2] = 1.0 / 22.0; ddn[2] = 1.0 / 22.0; }
+        else if (be) { ddn[1] = 0; dup[1] = 0; }
+        else { ddn[1] = 0; dup[1] = 2 * al; }
+        break;
+    case ORC_STG_D1_C2C:  /* ComputeTriD1_C2C :102-160 */
+        if (ts) {
+            dup[N] = w1 * 0; dup[N - 1] = w2 * alpha_p; dup[N - 2] = w3 * alpha_pp;
+            dg[N] = w1; dg[N - 1] = w2; dg[N - 2] = w3;
+            ddn[N] = w1 * SG_ALPHA; ddn[N - 1] = w2 * alpha_p; ddn[N - 2] = w3 * alpha_pp;
+        } else dg[N] = te ? 1 - al : 1 + al;
+        if (bs) {
+            ddn[1] = w1 * 0; ddn[2] = w2 * alpha_p; ddn[3] = w3 * alpha_pp;
+            dg[1] = w1; dg[2] = w2; dg[3] = w3;
+            dup[1] = w1 * SG_ALPHA; dup[2] = w2 * alpha_p; dup[3] = w3 * alpha_pp;
+        } else dg[1] = be ? 1 - al : 1 + al;
+        break;
+    case ORC_STG_D1_E2E:  /* ComputeTriD1_E2E :162-195 */
+        if (te) { dg[N] = 1; ddn[N] = 0; } else { dg[N] = 1; ddn[N] = 2 * al; }
+        if (be) { dg[1] = 1; dup[1] = 0; } else { dg[1] = 1; dup[1] = 2 * al; }
+        break;
+    case ORC_STG_IN_C2E: {  /* ComputeTriInterp_C2E :197-240 */
+        const double alpha1 = 1.0 / 6.0;
+        if (ts) { dup[N] = 0; ddn[N] = 0; dup[N - 1] = alpha1; ddn[N - 1] = alpha1; }
+        else if (te) { ddn[N] = 2 * al; dg[N] = 1; }
+        else { ddn[N] = 0; dg[N] = 1; }
+        if (bs) { dup[1] = 0; ddn[1] = 0; dg[1] = 1; dup[2] = alpha1; ddn[2] = alpha1; dg[2] = 1; }
+        else if (be) { dup[1] = 2 * al; dg[1] = 1; }
+        else { dup[1] = 0; dg[1] = 1; }
+        break;
+    }
+    case ORC_STG_IN_E2C: {  /* ComputeTriInterp_E2C :243-285 */
+        const double alpha0 = 1.0;
+        if (ts) { ddn[N] = alpha0; dup[N] = 0; }
+        else dg[N] = te ? 1 + al : 1 - al;
+        if (bs) { dup[1] = alpha0; ddn[1] = 0; }
+        else dg[1] = be ? 1 + al : 1 - al;
+        break;
+    }
+    case ORC_STG_D2_E2E:  /* ComputeTriD2_E2E :288-320 */
+        if (te) { dg[N] = 1; ddn[N] = 2 * al; } else { dg[N] = 1; ddn[N] = 0; }
+        if (be) { dg[1] = 1; dup[1] = 2 * al; } else { dg[1] = 1; dup[1] = 0; }
+        break;
+    case ORC_STG_D2_C2C:  /* ComputeTriD2_C2C :322-354 */
+        dg[N] = te ? 1 + al : 1 - al;
+        dg[1] = be ? 1 + al : 1 - al;
+        break;
+    }
+    return N;
+}
+
+/* RHS of the non-periodic staggered operators, 1-based f and r
+ * (STAGG_CD06_files: D1RHS_*, D2RHS_*, InterpRHS_* common files). */
+static void stg_rhs(int op, int n, double dz, int te, int be, int ts, int bs, const double *f, double *r)
+{
+    const double ob = 1.0 / dz, ob2 = ob / dz;
+    const int nE = n + 1;
+    const double w0s = 223.0 / 186.0, w1s = 61.0 / 62.0;
+    switch (op) {
+    case ORC_STG_D1_E2C: {  /* D1RHS_E2C_common.F90 */
+        const double a = (63.0 / 62.0), b = (17.0 / 62.0) / 3.0;
+        const double alpha1 = 37.0 / 183.0, alpha0 = -1.0;
+        const double a0 = (1.0 / 24.0) * (alpha0 - 23.0), b0 = (1.0 / 8.0) * (-9.0 * alpha0 + 7.0);
+        const double c0 = (1.0 / 8.0) * (9.0 * alpha0 + 1.0), d0 = -(1.0 / 24.0) * (alpha0 + 1.0);
+        const double a1 = (3.0 / 8.0) * (3.0 - 2.0 * alpha1), b1 = (1.0 / 8.0) * (-1.0 + 22.0 * alpha1);
+        const double A = a * ob, B = b * ob;
+        for (int k = 2; k <= n - 1; k++) r[k] = B * (f[k + 2] - f[k - 1]) + A * (f[k + 1] - f[k]);
+        if (bs) {
+            r[1] = w0s * ob * (a0 * f[1] + b0 * f[2] + c0 * f[3] + d0 * f[4]);
+            r[2] = w1s * ob * ((-b1 / 3.0) * f[1] + (-a1) * f[2] + a1 * f[3] + (b1 / 3.0) * f[4]);
+        } else if (be) r[1] = B * (f[3] - f[2]) + A * (f[2] - f[1]);
+        else r[1] = B * (f[3] + f[2]) + A * (f[2] - f[1]);
+        if (ts) {
+            r[n] = -w0s * ob * (a0 * f[nE] + b0 * f[nE - 1] + c0 * f[nE - 2] + d0 * f[nE - 3]);
+            r[n - 1] = w1s * ob * ((-b1 / 3.0) * f[nE - 3] + (-a1) * f[nE - 2] + a1 * f[nE - 1] + (b1 / 3.0) * f[nE]);
+        } else if (te) r[n] = B * (f[nE - 1] - f[nE - 2]) + A * (f[nE] - f[nE - 1]);
+        else r[n] = -B * (f[nE - 1] + f[nE - 2]) + A * (f[nE] - f[nE - 1]);
+        break;
+    }
+    case ORC_STG_D1_C2E: {  /* D1RHS_C2E_common.F90 */
+        const double a = (63.0 / 62.0), b = (17.0 / 62.0) / 3.0;
+        const double a0 = -71.0 / 24.0, b0 = 47.0 / 8.0, c0 = -31.0 / 8.0, d0 = 23.0 / 24.0, a1 = 12.0 / 11.0;
+        const double A = a * ob, B = b * ob;
+        for (int k = 2; k <= nE - 1; k++) r[k] = A * (f[k] - f[k - 1]);
+        for (int k = 3; k <= nE - 2; k++) r[k] += B * (f[k + 1] - f[k - 2]);
+        if (bs) {
+            r[1] = (a0 * f[1] + b0 * f[2] + c0 * f[3] + d0 * f[4]) * ob;
+            r[2] = (f[2] - f[1]) * (a1 * ob);
+        } else if (be) { r[1] = 0; r[2] += B * (f[3] - f[1]); }
+        else { r[1] = 2 * B * f[2] + 2 * A * f[1]; r[2] += B * (f[3] + f[1]); }
+        if (ts) {
+            r[nE - 1] = (f[n] - f[n - 1]) * (a1 * ob);
+            r[nE] = (a0 * f[n] + b0 * f[n - 1] + c0 * f[n - 2] + d0 * f[n - 3]) * (-ob);
+        } else if (te) { r[nE - 1] += B * (f[n] - f[n - 2]); r[nE] = 0; }
+        else { r[nE - 1] -= B * (f[n] + f[n - 2]); r[nE] = -2 * B * f[n - 1] - 2 * A * f[n]; }
+        break;
+    }
+    case ORC_STG_D1_C2C: {  /* D1RHS_C2C_common.F90 */
+        double w1, w2, w3, alpha_p, q_p, alpha_pp, q_pp, r_pp;
+        stg_lele(&w1, &w2, &w3, &alpha_p, &q_p, &alpha_pp, &q_pp, &r_pp);
+        const double A = a06d1 * ob, B = b06d1 * ob;
+        const double a3 = w3 * q_pp * ob, b3 = w3 * r_pp * ob, a2 = w2 * q_p * ob;
+        const double a_1 = w1 * (-SG_P * ob), b_1 = w1 * (SG_Q * ob), c_1 = w1 * (SG_R * ob), d_1 = w1 * (SG_S * ob);
+        if (bs) {
+            r[1] = a_1 * f[1] + b_1 * f[2] + c_1 * f[3] + d_1 * f[4];
+            r[2] = a2 * (f[3] - f[1]);
+            r[3] = a3 * (f[4] - f[2]) + b3 * (f[5] - f[1]);
+        } else if (be) {
+            r[1] = B * (f[3] - f[2]) + A * (f[2] - f[1]);
+            r[2] = B * (f[4] - f[1]) + A * (f[3] - f[1]);
+            r[3] = B * (f[5] - f[1]) + A * (f[4] - f[2]);
+        } else {
+            r[1] = B * (f[3] + f[2]) + A * (f[2] + f[1]);
+            r[2] = B * (f[4] + f[1]) + A * (f[3] - f[1]);
+            r[3] = B * (f[5] - f[1]) + A * (f[4] - f[2]);
+        }
+        for (int k = 4; k <= n - 3; k++) r[k] = B * (f[k + 2] - f[k - 2]) + A * (f[k + 1] - f[k - 1]);
+        if (ts) {
+            r[n - 2] = a3 * (f[n - 1] - f[n - 3]) + b3 * (f[n] - f[n - 4]);
+            r[n - 1] = a2 * (f[n] - f[n - 2]);
+            r[n] = -a_1 * f[n] - b_1 * f[n - 1] - c_1 * f[n - 2] - d_1 * f[n - 3];
+        } else if (te) {
+            r[n - 2] = B * (f[n] - f[n - 4]) + A * (f[n - 1] - f[n - 3]);
+            r[n - 1] = B * (f[n] - f[n - 3]) + A * (f[n] - f[n - 2]);
+            r[n] = B * (f[n - 1] - f[n - 2]) + A * (f[n] - f[n - 1]);
+        } else {
+            r[n - 2] = B * (f[n] - f[n - 4]) + A * (f[n - 1] - f[n - 3]);
+            r[n - 1] = -B * (f[n] + f[n - 3]) + A * (f[n] - f[n - 2]);
+            r[n] = -B * (f[n - 1] + f[n - 2]) - A * (f[n] + f[n - 1]);
+        }
+        break;
+    }
+    case ORC_STG_D1_E2E: {  /* D1RHS_E2E_common.F90 */
+        const double A = a06d1 * ob, B = b06d1 * ob;
+        for (int k = 2; k <= nE - 1; k++) r[k] = A * (f[k + 1] - f[k - 1]);
+        for (int k = 3; k <= nE - 2; k++) r[k] += B * (f[k + 2] - f[k - 2]);
+        if (be) { r[1] = 0; r[2] += B * (f[4] - f[2]); }
+        else { r[1] = B * (f[3] + f[3]) + A * (f[2] + f[2]); r[2] += B * (f[4] + f[2]); }
+        if (te) { r[nE] = 0; r[nE - 1] += B * (f[nE - 1] - f[nE - 3]); }
+        else {
+            r[nE] = -B * (f[nE - 2] + f[nE - 2]) - A * (f[nE - 1] + f[nE - 1]);
+            r[nE - 1] -= B * (f[nE - 1] + f[nE - 3]);
+        }
+        break;
+    }
+    case ORC_STG_IN_E2C: {  /* InterpRHS_E2C_common.F90 */
+        const double b = (1.0 / 10.0) / 2.0, a = (3.0 / 2.0) / 2.0, alpha0 = 1.0;
+        const double a0 = (1.0 / 16.0) * (5.0 - alpha0), b0 = (1.0 / 16.0) * (9.0 * alpha0 + 15.0);
+        const double c0 = (1.0 / 16.0) * (9.0 * alpha0 - 5.0), d0 = (1.0 / 16.0) * (1.0 - alpha0);
+        if (bs) r[1] = a0 * f[1] + b0 * f[2] + c0 * f[3] + d0 * f[4];
+        else if (be) r[1] = b * (f[3] + f[2]) + a * (f[2] + f[1]);
+        else r[1] = b * (f[3] - f[2]) + a * (f[2] + f[1]);
+        for (int k = 2; k <= n - 1; k++) r[k] = b * (f[k + 2] + f[k - 1]) + a * (f[k + 1] + f[k]);
+        if (ts) r[n] = a0 * f[nE] + b0 * f[nE - 1] + c0 * f[nE - 2] + d0 * f[nE - 3];
+        else if (te) r[n] = b * (f[nE - 1] + f[nE - 2]) + a * (f[nE] + f[nE - 1]);
+        else r[n] = b * (-f[nE - 1] + f[nE - 2]) + a * (f[nE] + f[nE - 1]);
+        break;
+    }
+    case ORC_STG_IN_C2E: {  /* InterpRHS_C2E_common.F90 */
+        const double b = (1.0 / 10.0) / 2.0, a = (3.0 / 2.0) / 2.0;
+        const double a0 = 15.0 / 8.0, b0 = -5.0 / 4.0, c0 = 3.0 / 8.0;
+        const double alpha1 = 1.0 / 6.0, a1 = (1.0 / 8.0) * (9.0 + 10.0 * alpha1);
+        for (int k = 2; k <= nE - 1; k++) r[k] = a * (f[k] + f[k - 1]);
+        for (int k = 3; k <= nE - 2; k++) r[k] += b * (f[k + 1] + f[k - 2]);
+        if (bs) { r[1] = a0 * f[1] + b0 * f[2] + c0 * f[3]; r[2] = (a1 / 2.0) * (f[2] + f[1]); }
+        else if (be) { r[1] = 2 * b * f[2] + 2 * a * f[1]; r[2] += b * (f[3] + f[1]); }
+        else { r[1] = 0; r[2] += b * (f[3] - f[1]); }
+        if (ts) { r[nE] = a0 * f[n] + b0 * f[n - 1] + c0 * f[n - 2]; r[nE - 1] = (a1 / 2.0) * (f[n] + f[n - 1]); }
+        else if (te) { r[nE] = 2 * b * f[n - 1] + 2 * a * f[n]; r[nE - 1] += b * (f[n] + f[n - 2]); }
+        else { r[nE] = 0; r[nE - 1] += b * (-f[n] + f[n - 2]); }
+        break;
+    }
+    case ORC_STG_D2_C2C: {  /* D2RHS_C2C_common.F90 */
+        const double A = (12.0 / 11.0) * ob2, B = ((3.0 / 11.0) / 4.0) * ob2;
+        for (int k = 2; k <= n - 1; k++) r[k] = A * (f[k + 1] + f[k - 1]);
+        for (int k = 3; k <= n - 2; k++) r[k] += B * (f[k + 2] + f[k - 2]);
+        if (be) { r[1] = B * (f[3] + f[2]) + A * (f[2] + f[1]); r[2] += B * (f[4] + f[1]); }
+        else { r[1] = B * (f[3] - f[2]) + A * (f[2] - f[1]); r[2] += B * (f[4] - f[1]); }
+        if (te) { r[n] = B * (f[n - 1] + f[n - 2]) + A * (f[n] + f[n - 1]); r[n - 1] += B * (f[n] + f[n - 3]); }
+        else { r[n] = B * (-f[n - 1] + f[n - 2]) + A * (-f[n] + f[n - 1]); r[n - 1] += B * (-f[n] + f[n - 3]); }
+        for (int k = 1; k <= n; k++) r[k] -= 2 * (B + A) * f[k];
+        break;
+    }
+    case ORC_STG_D2_E2E: {  /* D2RHS_E2E_common.F90 */
+        const double A = (12.0 / 11.0) * ob2, B = ((3.0 / 11.0) / 4.0) * ob2;
+        for (int k = 2; k <= nE - 1; k++) r[k] = A * (f[k + 1] + f[k - 1]);
+        for (int k = 3; k <= nE - 2; k++) r[k] += B * (f[k + 2] + f[k - 2]);
+        if (be) { r[1] = B * (f[3] + f[3]) + A * (f[2] + f[2]); r[2] += B * (f[4] + f[2]); }
+        else { r[1] = 0; r[2] += B * (f[4] - f[2]); }
+        if (te) { r[nE] = 2 * B * f[nE - 2] + 2 * A * f[nE - 1]; r[nE - 1] += B * (f[nE - 1] + f[nE - 3]); }
+        else { r[nE] = 0; r[nE - 1] += B * (-f[nE - 1] + f[nE - 3]); }
+        for (int k = 1; k <= nE; k++) r[k] -= 2 * (B + A) * f[k];
+        break;
+    }
+    }
+}
+
+/* SolveZTriREAL (TridiagSolver_allRoutines.F90:1-16) on one pencil with the
+ * factors (ddn*den, den, cp) of ComputeTri_common.F90. */
+static void stg_solve(int N, const double *ddn, const double *dg, const double *dup, double *y)
+{
+    double *cp = malloc(sizeof(double) * (N + 2)), *den = malloc(sizeof(double) * (N + 2));
+    cp[1] = dup[1] / dg[1];
+    for (int i = 2; i <= N - 1; i++) cp[i] = dup[i] / (dg[i] - ddn[i] * cp[i - 1]);
+    den[1] = 1.0 / dg[1];
+    for (int i = 2; i <= N; i++) den[i] = 1.0 / (dg[i] - ddn[i] * cp[i - 1]);
+    y[1] = y[1] * den[1];
+    for (int k = 2; k <= N; k++) y[k] = y[k] * den[k] - y[k - 1] * (ddn[k] * den[k]);
+    for (int k = N - 1; k >= 1; k--) y[k] = y[k] - cp[k] * y[k + 1];
+    free(cp);
+    free(den);
+}
+
+static int stg_nin(int op, int n)
+{
+    return (op == ORC_STG_D1_E2C || op == ORC_STG_IN_E2C || op == ORC_STG_D1_E2E || op == ORC_STG_D2_E2E) ? n + 1 : n;
+}
+
+/* Apply a non-periodic cd06stagg operator along z of f(n1, n2, n_in); flags
+ * bit 0 isTopEven, 1 isBotEven, 2 isTopSided, 3 isBotSided.  Returns 21 for
+ * n <= 4 (cd06stagg.F90:216-218). */
+int orc_stagg_np(int op, int n, double dz, int flags, int n1, int n2, const double *f, double *out)
+{
+    if (n <= 4) return 21;
+    const int te = flags & 1, be = (flags >> 1) & 1, ts = (flags >> 2) & 1, bs = (flags >> 3) & 1;
+    double *ddn = malloc(sizeof(double) * (n + 3)), *dg = malloc(sizeof(double) * (n + 3)),
+           *dup = malloc(sizeof(double) * (n + 3));
+    const int N = stg_tri(op, n, te, be, ts, bs, ddn, dg, dup);
+    if (N < 0) { free(ddn); free(dg); free(dup); return -1; }
+    const int nin = stg_nin(op, n);
+    const long pl = (long)n1 * n2;
+#pragma omp parallel
+    {
+        double *pin = malloc(sizeof(double) * (nin + 2)), *y = malloc(sizeof(double) * (N + 2));
+#pragma omp for schedule(static)
+        for (long p = 0; p < pl; p++) {
+            for (int k = 1; k <= nin; k++) pin[k] = f[p + (long)(k - 1) * pl];
+            stg_rhs(op, n, dz, te, be, ts, bs, pin, y);
+            stg_solve(N, ddn, dg, dup, y);
+            for (int k = 1; k <= N; k++) out[p + (long)(k - 1) * pl] = y[k];
+        }
+        free(pin);
+        free(y);
+    }
+    free(ddn); free(dg); free(dup);
+    return 0;
+}
+
+/* Dense (A, B) of a non-periodic staggered operator: A N x N, B N x n_in (row-major). */
+int orc_stagg_np_dense(int op, int n, double dz, int flags, double *A, double *B)
+{
+    if (n <= 4) return 21;
+    const int te = flags & 1, be = (flags >> 1) & 1, ts = (flags >> 2) & 1, bs = (flags >> 3) & 1;
+    double *ddn = malloc(sizeof(double) * (n + 3)), *dg = malloc(sizeof(double) * (n + 3)),
+           *dup = malloc(sizeof(double) * (n + 3));
+    const int N = stg_tri(op, n, te, be, ts, bs, ddn, dg, dup);
+    if (N < 0) { free(ddn); free(dg); free(dup); return -1; }
+    const int nin = stg_nin(op, n);
+    memset(A, 0, sizeof(double) * N * N);
+    for (int i = 1; i <= N; i++) {
+        if (i > 1) A[(i - 1) * N + i - 2] = ddn[i];
+        A[(i - 1) * N + i - 1] = dg[i];
+        if (i < N) A[(i - 1) * N + i] = dup[i];
+    }
+    double *e = calloc(nin + 2, sizeof(double)), *r = calloc(N + 2, sizeof(double));
+    for (int j = 1; j <= nin; j++) {
+        memset(e, 0, sizeof(double) * (nin + 2));
+        e[j] = 1.0;
+        stg_rhs(op, n, dz, te, be, ts, bs, e, r);
+        for (int i = 1; i <= N; i++) B[(i - 1) * nin + j - 1] = r[i];
+    }
+    free(e); free(r); free(ddn); free(dg); free(dup);
+    return 0;
+}

@@ -1,0 +1,138 @@
+"""TEST INFRASTRUCTURE ONLY (the checker): CPU restatement of the igrid spectral
+operators and of the periodic PadePoisson projection, in numpy.  Only tests/,
+__graft_entry__.smoke() and bench.py's CPU-baseline leg use this module; the
+product path never does.
+
+Arrays are numpy arrays indexed like the reference's Fortran arrays: real
+fields (nx, ny, nz), half spectra (nx/2+1, ny, nz) complex, edge fields with
+nz+1 planes.  The discrete Fourier transforms (FFTW in the reference,
+fft_3d.F90) are numpy.fft, whose unnormalized forward / 1/n inverse pair is
+the DFT FFTW computes; tests/test_oracle_pins.py checks it against a
+brute-force DFT.
+"""
+import numpy as np
+
+from . import STG_D1_C2E, STG_D1_E2C, apply, modk
+
+
+def getwavenums(n, dx):
+    """GetWaveNums (spectral.F90:1748-1763): (-pi + (i-1) 2 pi/(n - n%2))/dx, then
+    ifftshift: (0, 1, .., n/2-1, -n/2, .., -1) 2 pi / (n dx) for even n."""
+    dummy = n - n % 2
+    k = (-np.pi + np.arange(n) * 2.0 * np.pi / dummy) / dx
+    return np.fft.ifftshift(k)
+
+
+class Spectral:
+    """type(spectral) with dimTransform = 2 and an x input pencil
+    (spectral.F90:867-1260; the periodic-z dealiasing of :755-865)."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, fix_oddball=False, dealias_fact=2.0 / 3.0, periodic_z=False):
+        self.shape = (nx, ny, nz)
+        self.nxh = nx // 2 + 1
+        self.periodic_z = periodic_z
+        k1 = getwavenums(nx, dx)
+        k2 = getwavenums(ny, dy)
+        k3 = getwavenums(nz, dz)
+        # STEP 3 (:1027-1033): the oddball takes a + sign
+        k1[nx // 2] = -k1[nx // 2]
+        k2[ny // 2] = -k2[ny // 2]
+        if nz > 1:
+            k3[nz // 2] = -k3[nz // 2]
+        self.k1 = k1[: self.nxh].copy()
+        self.k2 = k2.copy()
+        self.k3 = k3.copy()
+        # STEP 7 (:1135-1148): 2/3 rule on (k1, k2), before the oddball fix
+        kdx = ((2.0 / 3.0) * np.pi) / dx
+        kdy = ((2.0 / 3.0) * np.pi) / dy
+        self.g1 = (np.abs(self.k1) < kdx).astype(float)
+        self.g2 = (np.abs(self.k2) < kdy).astype(float)
+        self.g3 = np.ones(nz)
+        # STEP 11 (:1201-1216, two periodic directions): zero x and y oddballs
+        if fix_oddball:
+            self.k2[ny // 2] = 0.0
+            self.k1[nx // 2] = 0.0
+        if periodic_z:
+            # init_periodic_inZ_procedures (:783-813): mask of stored k1, k2, k3
+            self.g1 = np.where(np.abs(self.k1) >= (dealias_fact * np.pi) / dx, 0.0, 1.0)
+            self.g2 = np.where(np.abs(self.k2) >= (dealias_fact * np.pi) / dy, 0.0, 1.0)
+            self.g3 = np.where(np.abs(self.k3) >= (dealias_fact * np.pi) / dz, 0.0, 1.0)
+
+    def fft(self, f):
+        """fft2_x2y (fft_3d.F90:645-663): r2c along x, c2c along y, unnormalized."""
+        return np.fft.fft(np.fft.rfft(np.asarray(f, dtype=np.float64), axis=0), axis=1)
+
+    def ifft(self, fhat, set_oddball=False):
+        """ifft2_y2x (fft_3d.F90:616-643): inverse c2c along y, optionally zero
+        the kx = nx/2 modes, c2r along x (real parts of kx = 0 and nx/2), times
+        1/(nx ny)."""
+        t = np.fft.ifft(np.asarray(fhat, dtype=np.complex128), axis=1)
+        if set_oddball:
+            t[self.shape[0] // 2] = 0.0
+        return np.fft.irfft(t, n=self.shape[0], axis=0)
+
+    def mtimes_ik(self, f, dir):
+        """mTimes_ik1/ik2 (spectral.F90:235-312): i k f."""
+        k = self.k1[:, None, None] if dir == 1 else self.k2[None, :, None]
+        f = np.asarray(f, dtype=np.complex128)
+        return (-k * f.imag) + 1j * (k * f.real)
+
+    def dealias(self, fhat):
+        """dealias (spectral.F90:314-341)."""
+        g = self.g1[:, None, None] * self.g2[None, :, None]
+        if not self.periodic_z:
+            return fhat * g
+        t = np.fft.fft(fhat, axis=2) * (g * self.g3[None, None, :])
+        return np.fft.ifft(t, axis=2)
+
+    def dealias_edge(self, fhat):
+        """dealias_edgeField (spectral.F90:343-363): nz+1 planes, periodic z."""
+        out = np.array(fhat, dtype=np.complex128)
+        if not self.periodic_z:
+            return out
+        nz = self.shape[2]
+        out[:, :, :nz] = self.dealias(out[:, :, :nz])
+        out[:, :, nz] = out[:, :, 0]
+        return out
+
+
+def _stagg(op, f, dz, nz):
+    """Periodic cd06stagg operator along z of a complex field (real operator
+    applied to the real and imaginary parts)."""
+    return apply(op, f.real, 3, dz, True, n=nz) + 1j * apply(op, f.imag, 3, dz, True, n=nz)
+
+
+class PadePoissonPeriodic:
+    """Periodic-in-z padepoisson with cd06 staggered z derivatives
+    (PadePoisson.F90:76-128, 130-180, 386-432, 1165-1244)."""
+
+    def __init__(self, sp, dx, dy, dz):
+        nx, ny, nz = sp.shape
+        self.sp, self.dz, self.nz = sp, dz, nz
+        k1 = getwavenums(nx, dx)[: sp.nxh]
+        k2 = getwavenums(ny, dy)
+        # cd06stagg modified wavenumber of k3 (getmodCD06stagg via derivZ)
+        k3m = modk(2, nz, dz)
+        ks = k1[:, None, None] ** 2 + k2[None, :, None] ** 2 + k3m[None, None, :] ** 2
+        self.kradsq_inv = np.where(ks <= 1e-14, 0.0, 1.0 / np.where(ks <= 1e-14, 1.0, ks))
+
+    def _div(self, uhat, vhat, what):
+        sp = self.sp
+        k1 = sp.k1[:, None, None]
+        k2 = sp.k2[None, :, None]
+        dwdz = _stagg(STG_D1_E2C, np.asarray(what), self.dz, self.nz)
+        return dwdz + 1j * (k1 * uhat + k2 * vhat)
+
+    def projection(self, uhat, vhat, what):
+        """PeriodicProjection: returns the projected (uhat, vhat, what)."""
+        sp = self.sp
+        f = self._div(uhat, vhat, what)
+        p = np.fft.ifft(-self.kradsq_inv * np.fft.fft(f, axis=2), axis=2)
+        dpdz = _stagg(STG_D1_C2E, p, self.dz, self.nz)
+        k1 = sp.k1[:, None, None]
+        k2 = sp.k2[None, :, None]
+        return uhat - 1j * k1 * p, vhat - 1j * k2 * p, what - dpdz
+
+    def divergence(self, uhat, vhat, what):
+        """DivergenceCheck steps 1-3: the real divergence field."""
+        return self.sp.ifft(self._div(uhat, vhat, what))

@@ -11,6 +11,8 @@ This module mirrors the reference's Fortran type-bound API for the path:
     type(cf90) filter1..3 (bc1, bcn)             cf90().filter1(f, fil, na, nb, bc1, bcn)
     type(cd06stagg) ddz_E2C ... InterpZ_C2E      cd06stagg().ddz_E2C(fE, dfC, n1, n2)
     type(PoissonPeriodic) poisson_solve          PoissonPeriodic().poisson_solve(rhs, f)
+    type(spectral) fft/ifft/mTimes_ik/dealias    spectral().fft(f, fhat) ...
+    type(padepoisson) PressureProjection         padepoisson().PressureProjection(uh, vh, wh)
 
 Field arguments are torch tensors resident on the GPU (PyTorch supplies the
 device memory and stream only), holding the reference's column-major layout
@@ -76,11 +78,16 @@ def _stream(stream):
 
 def _prec(f, prec):
     import torch
-    if f.dtype == torch.float64:
+    if f.dtype in (torch.float64, torch.complex128):
         return PDO_F64
-    if f.dtype == torch.float32:
+    if f.dtype in (torch.float32, torch.complex64):
         return PREC[prec] if prec != "f64" else PDO_F32_REFINE
-    raise TypeError("fields must be float64 or float32")
+    raise TypeError("fields must be float64/complex128 or float32/complex64")
+
+
+def _reals(t):
+    """Number of real words a (real or complex) tensor holds."""
+    return t.numel() * (2 if t.is_complex() else 1)
 
 
 def _check_sizes(f, out, n_in, n_out, na, nb):
@@ -214,11 +221,18 @@ class cf90(_Op):
 
 
 class cd06stagg(_Op):
-    """type(cd06stagg), src/derivatives/cd06stagg.F90 (periodic)."""
+    """type(cd06stagg), src/derivatives/cd06stagg.F90: init(n, dz) is the
+    periodic object, init(n, dz, isTopEven, isBotEven[, isTopSided,
+    isBotSided]) the non-periodic one (cd06stagg.F90:170-232)."""
 
-    def init(self, n, dz):
+    def init(self, n, dz, isTopEven=None, isBotEven=None, isTopSided=False, isBotSided=False):
         self.n = n
-        return _check(lib().pdo_cd06stagg_init_periodic(ctypes.byref(self._h), n, dz))
+        if isTopEven is None and isBotEven is None:
+            self.periodic = True
+            return _check(lib().pdo_cd06stagg_init_periodic(ctypes.byref(self._h), n, dz))
+        self.periodic = False
+        return _check(lib().pdo_cd06stagg_init(ctypes.byref(self._h), n, dz, int(bool(isTopEven)),
+                                               int(bool(isBotEven)), int(bool(isTopSided)), int(bool(isBotSided))))
 
     def _ap(self, name, fi, fo, n1, n2, n_in, n_out, is_complex, prec, stream):
         k = 2 if is_complex else 1
@@ -245,6 +259,14 @@ class cd06stagg(_Op):
 
     def InterpZ_C2E(self, fC, ifE, n1, n2, is_complex=False, prec="f32_refine", stream=None):
         return self._ap("pdo_cd06stagg_interpz_C2E", fC, ifE, n1, n2, self.n, self.n + 1, is_complex, prec, stream)
+
+    def ddz_C2C(self, fC, dfC, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        """non-periodic objects only"""
+        return self._ap("pdo_cd06stagg_ddz_C2C", fC, dfC, n1, n2, self.n, self.n, is_complex, prec, stream)
+
+    def ddz_E2E(self, fE, dfE, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        """non-periodic objects only"""
+        return self._ap("pdo_cd06stagg_ddz_E2E", fE, dfE, n1, n2, self.n + 1, self.n + 1, is_complex, prec, stream)
 
 
 class PoissonPeriodic:
@@ -306,6 +328,121 @@ class FullRHS:
     def destroy(self):
         if self._h:
             lib().pdo_rhs_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class spectral:
+    """type(spectral) of src/incompressible/spectral.F90 as igrid uses it:
+    2D (x, y) transforms of (nx, ny, nz) fields into (nx/2+1, ny, nz) spectra,
+    i k multiplication and dealiasing (see pdo_spectral_* in the C header)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def init(self, nx, ny, nz, dx, dy, dz, fixOddball=False, dealiasF=2.0 / 3.0, init_periodicInZ=False):
+        self.shape = (nx, ny, nz)
+        self.nxh = nx // 2 + 1
+        return _check(lib().pdo_spectral_init(ctypes.byref(self._h), nx, ny, nz, dx, dy, dz, int(fixOddball),
+                                              dealiasF, int(init_periodicInZ)))
+
+    def _nspec(self, planes=None):
+        return self.nxh * self.shape[1] * (self.shape[2] if planes is None else planes)
+
+    def fft(self, f, fhat, prec="f32_refine", stream=None):
+        if f.numel() < self.shape[0] * self.shape[1] * self.shape[2] or _reals(fhat) < 2 * self._nspec():
+            raise ValueError("buffer smaller than the declared shape")
+        return _check(lib().pdo_spectral_fft(self._h, _prec(f, prec), _ptr(f), _ptr(fhat), _stream(stream)))
+
+    def ifft(self, fhat, f, setOddball=False, prec="f32_refine", stream=None):
+        if f.numel() < self.shape[0] * self.shape[1] * self.shape[2] or _reals(fhat) < 2 * self._nspec():
+            raise ValueError("buffer smaller than the declared shape")
+        return _check(lib().pdo_spectral_ifft(self._h, _prec(f, prec), _ptr(fhat), _ptr(f), int(setOddball),
+                                              _stream(stream)))
+
+    def mTimes_ik(self, dir, fin, fout=None, prec="f32_refine", stream=None):
+        """mTimes_ik1 (dir 1) / mTimes_ik2 (dir 2); in place when fout is None."""
+        fout = fin if fout is None else fout
+        if _reals(fin) < 2 * self._nspec() or _reals(fout) < 2 * self._nspec():
+            raise ValueError("buffer smaller than the declared shape")
+        return _check(lib().pdo_spectral_mtimes_ik(self._h, _prec(fin, prec), dir, _ptr(fin), _ptr(fout),
+                                                   _stream(stream)))
+
+    def mTimes_ik1(self, fin, fout=None, **kw):
+        return self.mTimes_ik(1, fin, fout, **kw)
+
+    def mTimes_ik2(self, fin, fout=None, **kw):
+        return self.mTimes_ik(2, fin, fout, **kw)
+
+    def dealias(self, fhat, prec="f32_refine", stream=None):
+        if _reals(fhat) < 2 * self._nspec():
+            raise ValueError("buffer smaller than the declared shape")
+        return _check(lib().pdo_spectral_dealias(self._h, _prec(fhat, prec), _ptr(fhat), _stream(stream)))
+
+    def dealias_edgeField(self, fhat, prec="f32_refine", stream=None):
+        if _reals(fhat) < 2 * self._nspec(self.shape[2] + 1):
+            raise ValueError("buffer smaller than the declared shape")
+        return _check(lib().pdo_spectral_dealias_edge(self._h, _prec(fhat, prec), _ptr(fhat), _stream(stream)))
+
+    def wavenumbers(self):
+        """(k1, k2, k3, g1, g2, g3) host arrays; the dealias mask is g1 g2 g3."""
+        import numpy as np
+        nx, ny, nz = self.shape
+        arrs = [np.zeros(self.nxh), np.zeros(ny), np.zeros(nz), np.zeros(self.nxh), np.zeros(ny), np.zeros(nz)]
+        dp = ctypes.POINTER(ctypes.c_double)
+        _check(lib().pdo_spectral_wavenumbers(self._h, *[a.ctypes.data_as(dp) for a in arrs]))
+        return tuple(arrs)
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_spectral_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class padepoisson:
+    """Periodic-in-z type(padepoisson) of src/incompressible/PadePoisson.F90 with
+    cd06 staggered z derivatives (Pade6stagg scheme cd06)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def init(self, sp, dz):
+        self.sp = sp
+        return _check(lib().pdo_padepoisson_init_periodic(ctypes.byref(self._h), sp._h, dz))
+
+    def _sizes(self, uh, vh, wh):
+        n = self.sp._nspec()
+        ne = self.sp._nspec(self.sp.shape[2] + 1)
+        if _reals(uh) < 2 * n or _reals(vh) < 2 * n or _reals(wh) < 2 * ne:
+            raise ValueError("buffer smaller than the declared shape")
+
+    def PressureProjection(self, uhat, vhat, what, prec="f32_refine", stream=None):
+        self._sizes(uhat, vhat, what)
+        return _check(lib().pdo_padepoisson_projection(self._h, _prec(uhat, prec), _ptr(uhat), _ptr(vhat),
+                                                       _ptr(what), _stream(stream)))
+
+    def DivergenceCheck(self, uhat, vhat, what, divergence, prec="f32_refine", stream=None):
+        self._sizes(uhat, vhat, what)
+        nx, ny, nz = self.sp.shape
+        if divergence.numel() < nx * ny * nz:
+            raise ValueError("buffer smaller than the declared shape")
+        return _check(lib().pdo_padepoisson_divergence(self._h, _prec(uhat, prec), _ptr(uhat), _ptr(vhat),
+                                                       _ptr(what), _ptr(divergence), _stream(stream)))
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_padepoisson_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

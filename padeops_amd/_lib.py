@@ -55,6 +55,9 @@ SIGNATURES = {
     "pdo_cd06stagg_d2dz2_E2E": (_STAGG, _c_int),
     "pdo_cd06stagg_interpz_E2C": (_STAGG, _c_int),
     "pdo_cd06stagg_interpz_C2E": (_STAGG, _c_int),
+    "pdo_cd06stagg_init": ([_pvp, _c_int, _c_double, _c_int, _c_int, _c_int, _c_int], _c_int),
+    "pdo_cd06stagg_ddz_C2C": (_STAGG, _c_int),
+    "pdo_cd06stagg_ddz_E2E": (_STAGG, _c_int),
     "pdo_op_destroy": ([_vp], _c_int),
     "pdo_op_dense": ([_vp, _c_int, _c_int, _c_int, _pdbl, _pdbl], _c_int),
     "pdo_op_dims": ([_vp, _c_int, _pint, _pint], _c_int),
@@ -64,6 +67,19 @@ SIGNATURES = {
                                    _c_int, _c_int, _c_int, _c_int], _c_int),
     "pdo_poisson_periodic_solve": ([_vp, _c_int, _vp, _vp, _vp], _c_int),
     "pdo_poisson_destroy": ([_vp], _c_int),
+    "pdo_spectral_init": ([_pvp, _c_int, _c_int, _c_int, _c_double, _c_double, _c_double, _c_int, _c_double,
+                           _c_int], _c_int),
+    "pdo_spectral_fft": ([_vp, _c_int, _vp, _vp, _vp], _c_int),
+    "pdo_spectral_ifft": ([_vp, _c_int, _vp, _vp, _c_int, _vp], _c_int),
+    "pdo_spectral_mtimes_ik": ([_vp, _c_int, _c_int, _vp, _vp, _vp], _c_int),
+    "pdo_spectral_dealias": ([_vp, _c_int, _vp, _vp], _c_int),
+    "pdo_spectral_dealias_edge": ([_vp, _c_int, _vp, _vp], _c_int),
+    "pdo_spectral_wavenumbers": ([_vp] + [_pdbl] * 6, _c_int),
+    "pdo_spectral_destroy": ([_vp], _c_int),
+    "pdo_padepoisson_init_periodic": ([_pvp, _vp, _c_double], _c_int),
+    "pdo_padepoisson_projection": ([_vp, _c_int, _vp, _vp, _vp, _vp], _c_int),
+    "pdo_padepoisson_divergence": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "pdo_padepoisson_destroy": ([_vp], _c_int),
     "pdo_rhs_init": ([_pvp, _c_int, _c_double, _c_int], _c_int),
     "pdo_rhs_eval": ([_vp] + [_vp] * 7 + [_vp], _c_int),
     "pdo_rhs_pressure": ([_vp, _pvp], _c_int),

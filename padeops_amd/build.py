@@ -2,6 +2,9 @@
 
 Every source under padeops_amd/csrc is compiled (host C++ and HIP alike) by
 hipcc for --offload-arch=gfx950 and linked into padeops_amd/libpadeops_hip.so.
+When amdflang is present, the Fortran ISO_C_BINDING shim
+(padeops_amd/fortran/padeops_hip_mod.F90) and its driver test_shim are built
+too, against that library.
 Usage:  python -m padeops_amd.build   (or padeops_amd.build.build()).
 """
 import concurrent.futures as cf
@@ -40,6 +43,43 @@ def _compile(src):
     return obj
 
 
+FORT = os.path.join(HERE, "fortran")
+
+
+def amdflang():
+    for c in (os.environ.get("PDO_FC"), "/opt/rocm/bin/amdflang", shutil.which("amdflang")):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def build_fortran(verbose=False):
+    """Compile the shim module and link the driver padeops_amd/fortran/test_shim
+    (rpath to the library one directory up).  Returns its path, or None when
+    no Fortran compiler is installed."""
+    fc = amdflang()
+    if fc is None:
+        return None
+    work = os.path.join(OBJ, "fortran")
+    os.makedirs(work, exist_ok=True)
+    mod = os.path.join(FORT, "padeops_hip_mod.F90")
+    drv = os.path.join(FORT, "test_shim.F90")
+    exe = os.path.join(FORT, "test_shim")
+    steps = [
+        [fc, "-O2", "-module-dir", work, "-c", mod, "-o", os.path.join(work, "mod.o")],
+        [fc, "-O2", "-module-dir", work, "-c", drv, "-o", os.path.join(work, "drv.o")],
+        [fc, os.path.join(work, "drv.o"), os.path.join(work, "mod.o"), "-L", HERE, "-lpadeops_hip",
+         "-Wl,-rpath,$ORIGIN/..", "-o", exe],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("Fortran build failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
+    if verbose:
+        print("built", exe)
+    return exe
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".cpp", ".hip")))
@@ -55,6 +95,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran(verbose)
     return LIB
 
 

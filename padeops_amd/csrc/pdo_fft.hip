@@ -214,8 +214,8 @@ template <> struct CLoad<float, double> : CLoadF<double> {};
 // MODE 0 forward, 1 inverse, 2 forward -> Poisson multiply -> inverse.
 // ---------------------------------------------------------------------------
 template <typename TS, typename TC, int MODE, int L>
-__global__ void __launch_bounds__(1024) k_fft_strided(TS *__restrict__ data, long na, long sb, int se,
-                                                     const TC *__restrict__ twL, int lgtx, PoissonMul<TC> pm)
+__global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, long na, long sb, int se,
+                                                     const TC *__restrict__ twL, int lgtx, SpecMul pm)
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
@@ -223,26 +223,33 @@ __global__ void __launch_bounds__(1024) k_fft_strided(TS *__restrict__ data, lon
     const int ln = threadIdx.x & (TX - 1), jb = threadIdx.x >> lgtx;
     const long a = (long)blockIdx.x * TX + ln;
     const bool act = a < na;
-    TS *const line0 = data + 2 * ((act ? a : 0) + (long)blockIdx.y * sb);
+    const long off = 2 * ((act ? a : 0) + (long)blockIdx.y * sb);
+    const TS *const lin = src + off;
+    TS *const lout = dst + off;
     const C2<TC> *tw = (const C2<TC> *)twL;
     using IO = CLoad<TS, TC>;
-    auto ldg = [&](int i) { return IO::ld(line0, i * se); };
-    auto stg = [&](int i, C2<TC> v) { IO::st(line0, i * se, v); };
+    auto ldg = [&](int i) { return IO::ld(lin, i * se); };
+    auto stg = [&](int i, C2<TC> v) { IO::st(lout, i * se, v); };
     auto buf = [&](int i) -> C2<TC> & { return tile[(i << lgtx) + ln]; };
     if constexpr (MODE == 0) {
         line_fft<TC, -1, false, L>(ldg, stg, buf, tw, jb, act);
     } else if constexpr (MODE == 1) {
         line_fft<TC, +1, false, L>(ldg, stg, buf, tw, jb, act);
     } else {
-        // forward into LDS, scale by -1/|k'|^2, inverse from LDS to HBM;
-        // the line is (kx, ky) = (a, blockIdx.y)
+        // forward into LDS, multiply by the spectral factor of (kx, ky, kz) =
+        // (a, blockIdx.y, i), inverse from LDS to HBM
         const int kx = (int)a, ky = (int)blockIdx.y;
-        const double kxy = act ? pm.kx2[kx] + pm.ky2[ky] : 0.0;
+        const double fxy = act ? (pm.kind == 2 ? pm.ax[kx] * pm.ay[ky] : pm.ax[kx] + pm.ay[ky]) : 0.0;
         auto sts = [&](int i, C2<TC> v) {
-            const double ks = kxy + pm.kz2[i];
             double fac;
-            if (pm.zero_policy == 0) fac = (kx == 0 && ky == 0 && i == 0) ? 0.0 : -1.0 / (ks + 1.0e-20);
-            else fac = (ks <= 1.0e-14) ? 0.0 : -1.0 / ks;
+            if (pm.kind == 2) {
+                fac = fxy * pm.az[i];
+            } else {
+                const double ks = fxy + pm.az[i];
+                if (pm.kind == 0) fac = (kx == 0 && ky == 0 && i == 0) ? 0.0 : -1.0 / (ks + 1.0e-20);
+                else fac = (ks <= 1.0e-14) ? 0.0 : -1.0 / ks;
+            }
+            fac *= pm.scale;
             v.x *= (TC)fac;
             v.y *= (TC)fac;
             buf(i) = v;
@@ -290,15 +297,17 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c(const TS *__restrict__ in, T
     }
 }
 
-// pass 5: x complex-to-real, scaled by `scale`
+// pass 5: x complex-to-real, scaled by `scale`.  Like a Hermitian c2r, only
+// the real parts of the kx = 0 and kx = nx/2 modes enter; with oddball != 0
+// the kx = nx/2 mode is dropped altogether.
 template <typename TS, typename TC, int L>
 __global__ void __launch_bounds__(1024) k_fft_x_c2r(const TS *__restrict__ in, TS *__restrict__ out, long nlines,
                                                    const TC *__restrict__ twL, const TC *__restrict__ twN, int TXL,
-                                                   TC scale, int nxp)
+                                                   TC scale, int nxp, int oddball)
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
-    constexpr int nx = 2 * L, L8 = L >> 3, LDL = L + 1, nxh = L + 1;
+    constexpr int nx = 2 * L, L8 = L >> 3, LDL = L + 1;
     const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
     const long l = (long)blockIdx.x * TXL + line;
     const bool act = (line < TXL) && (l < nlines);
@@ -306,16 +315,30 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r(const TS *__restrict__ in, T
     const TS *src = in + (act ? l : 0) * (long)nxp * 2;
     auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + i]; };
     const C2<TC> *wN = (const C2<TC> *)twN;
-    // stage the nx/2+1 modes of the line in LDS (coalesced), then form
-    // Z_k = E_k + i O_k in place, one thread per pair (k, L-k):
+    // stage the nx/2+1 modes of the line in LDS (coalesced, all loads of a
+    // thread in flight together), then form Z_k = E_k + i O_k in place, one
+    // thread per pair (k, L-k):
     // E = (X_k + conj X_{L-k})/2, O = (X_k - conj X_{L-k}) conj(W^k)/2
-    if (act)
-        for (int k = jb; k < nxh; k += L8) buf(k) = IO::ld(src, k);
+    if (act) {
+        C2<TC> v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = IO::ld(src, jb + q * L8);
+        C2<TC> vl = {};
+        if (jb == 0) vl = IO::ld(src, L);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) buf(jb + q * L8) = v[q];
+        if (jb == 0) buf(L) = vl;
+    }
     __syncthreads();
     if (act) {
         for (int k = jb; k <= L / 2; k += L8) {
             const int kc = L - k;
-            const C2<TC> xk = buf(k), xc = buf(kc);
+            C2<TC> xk = buf(k), xc = buf(kc);
+            if (k == 0) {
+                xk.y = 0;
+                xc.y = 0;
+                if (oddball) xc.x = 0;
+            }
             auto z = [&](C2<TC> a, C2<TC> b, int kk) {
                 const C2<TC> ev = {(TC)0.5 * (a.x + b.x), (TC)0.5 * (a.y - b.y)};
                 const C2<TC> df = {(TC)0.5 * (a.x - b.x), (TC)0.5 * (a.y + b.y)};
@@ -350,8 +373,8 @@ static int ilog2(int v)
 }
 
 template <typename TS, typename TC, int MODE, int L>
-static void launch_strided_fft(TS *spec, long na, long sb, long se, const TC *tw, const PoissonMul<TC> &pm, int nb,
-                               hipStream_t st)
+static void launch_strided_fft(const TS *src, TS *dst, long na, long sb, long se, const TC *tw, const SpecMul &pm,
+                               int nb, hipStream_t st)
 {
     const size_t ce = 2 * sizeof(TC);
     int TX = 1024 / (L / 8);
@@ -359,29 +382,29 @@ static void launch_strided_fft(TS *spec, long na, long sb, long se, const TC *tw
     int lg = 0;
     while ((1 << lg) < TX) lg++;
     dim3 grid((unsigned)((na + TX - 1) / TX), (unsigned)nb);
-    hipLaunchKernelGGL((k_fft_strided<TS, TC, MODE, L>), grid, dim3(TX * (L / 8)), (size_t)TX * L * ce, st, spec, na,
-                       sb, (int)se, tw, lg, pm);
+    hipLaunchKernelGGL((k_fft_strided<TS, TC, MODE, L>), grid, dim3(TX * (L / 8)), (size_t)TX * L * ce, st, src, dst,
+                       na, sb, (int)se, tw, lg, pm);
 }
 
 template <typename TS, typename TC, int MODE>
-static void strided_fft(int L, TS *spec, long na, long sb, long se, const TC *tw, const PoissonMul<TC> &pm, int nb,
-                        hipStream_t st)
+static void strided_fft(int L, const TS *src, TS *dst, long na, long sb, long se, const TC *tw, const SpecMul &pm,
+                        int nb, hipStream_t st)
 {
     switch (L) {
-    case 8: launch_strided_fft<TS, TC, MODE, 8>(spec, na, sb, se, tw, pm, nb, st); break;
-    case 16: launch_strided_fft<TS, TC, MODE, 16>(spec, na, sb, se, tw, pm, nb, st); break;
-    case 32: launch_strided_fft<TS, TC, MODE, 32>(spec, na, sb, se, tw, pm, nb, st); break;
-    case 64: launch_strided_fft<TS, TC, MODE, 64>(spec, na, sb, se, tw, pm, nb, st); break;
-    case 128: launch_strided_fft<TS, TC, MODE, 128>(spec, na, sb, se, tw, pm, nb, st); break;
-    case 256: launch_strided_fft<TS, TC, MODE, 256>(spec, na, sb, se, tw, pm, nb, st); break;
-    case 512: launch_strided_fft<TS, TC, MODE, 512>(spec, na, sb, se, tw, pm, nb, st); break;
-    case 1024: launch_strided_fft<TS, TC, MODE, 1024>(spec, na, sb, se, tw, pm, nb, st); break;
+    case 8: launch_strided_fft<TS, TC, MODE, 8>(src, dst, na, sb, se, tw, pm, nb, st); break;
+    case 16: launch_strided_fft<TS, TC, MODE, 16>(src, dst, na, sb, se, tw, pm, nb, st); break;
+    case 32: launch_strided_fft<TS, TC, MODE, 32>(src, dst, na, sb, se, tw, pm, nb, st); break;
+    case 64: launch_strided_fft<TS, TC, MODE, 64>(src, dst, na, sb, se, tw, pm, nb, st); break;
+    case 128: launch_strided_fft<TS, TC, MODE, 128>(src, dst, na, sb, se, tw, pm, nb, st); break;
+    case 256: launch_strided_fft<TS, TC, MODE, 256>(src, dst, na, sb, se, tw, pm, nb, st); break;
+    case 512: launch_strided_fft<TS, TC, MODE, 512>(src, dst, na, sb, se, tw, pm, nb, st); break;
+    case 1024: launch_strided_fft<TS, TC, MODE, 1024>(src, dst, na, sb, se, tw, pm, nb, st); break;
     }
 }
 
 template <typename TS, typename TC, int L>
 static void launch_x_fft(bool inverse, const TS *in, TS *out, long nlines, const TC *tw, const TC *twn, TC scale,
-                         int nxp, hipStream_t st)
+                         int nxp, int oddball, hipStream_t st)
 {
     const size_t ce = 2 * sizeof(TC);
     int TXL = 1024 / (L / 8);
@@ -392,44 +415,52 @@ static void launch_x_fft(bool inverse, const TS *in, TS *out, long nlines, const
                            out, nlines, tw, twn, TXL, nxp);
     else
         hipLaunchKernelGGL((k_fft_x_c2r<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * (L + 1) * ce, st, in,
-                           out, nlines, tw, twn, TXL, scale, nxp);
+                           out, nlines, tw, twn, TXL, scale, nxp, oddball);
 }
 
 template <typename TS, typename TC>
 static void x_fft(int L, bool inverse, const TS *in, TS *out, long nlines, const TC *tw, const TC *twn, TC scale,
-                  int nxp, hipStream_t st)
+                  int nxp, int oddball, hipStream_t st)
 {
     switch (L) {
-    case 8: launch_x_fft<TS, TC, 8>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
-    case 16: launch_x_fft<TS, TC, 16>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
-    case 32: launch_x_fft<TS, TC, 32>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
-    case 64: launch_x_fft<TS, TC, 64>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
-    case 128: launch_x_fft<TS, TC, 128>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
-    case 256: launch_x_fft<TS, TC, 256>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
-    case 512: launch_x_fft<TS, TC, 512>(inverse, in, out, nlines, tw, twn, scale, nxp, st); break;
+    case 8: launch_x_fft<TS, TC, 8>(inverse, in, out, nlines, tw, twn, scale, nxp, oddball, st); break;
+    case 16: launch_x_fft<TS, TC, 16>(inverse, in, out, nlines, tw, twn, scale, nxp, oddball, st); break;
+    case 32: launch_x_fft<TS, TC, 32>(inverse, in, out, nlines, tw, twn, scale, nxp, oddball, st); break;
+    case 64: launch_x_fft<TS, TC, 64>(inverse, in, out, nlines, tw, twn, scale, nxp, oddball, st); break;
+    case 128: launch_x_fft<TS, TC, 128>(inverse, in, out, nlines, tw, twn, scale, nxp, oddball, st); break;
+    case 256: launch_x_fft<TS, TC, 256>(inverse, in, out, nlines, tw, twn, scale, nxp, oddball, st); break;
+    case 512: launch_x_fft<TS, TC, 512>(inverse, in, out, nlines, tw, twn, scale, nxp, oddball, st); break;
     }
 }
+
+template <typename TC> struct Tw {
+    const TC *x, *xn, *y, *z;
+    explicit Tw(const PoissonPlanDev &pl)
+    {
+        const bool dbl = sizeof(TC) == 8;
+        x = (const TC *)(dbl ? (const void *)pl.tw_d[0] : (const void *)pl.tw_f[0]);
+        xn = (const TC *)(dbl ? (const void *)pl.twn_d : (const void *)pl.twn_f);
+        y = (const TC *)(dbl ? (const void *)pl.tw_d[1] : (const void *)pl.tw_f[1]);
+        z = (const TC *)(dbl ? (const void *)pl.tw_d[2] : (const void *)pl.tw_f[2]);
+    }
+};
 
 template <typename TS, typename TC>
 static hipError_t run_poisson_t(const PoissonPlanDev &pl, const TS *rhs, TS *f, TS *spec, hipStream_t st)
 {
     const int nx = pl.nx, ny = pl.ny, nz = pl.nz, nxh = nx / 2 + 1, L = nx / 2;
-    const bool dbl = sizeof(TC) == 8;
-    const TC *twx = (const TC *)(dbl ? (const void *)pl.tw_d[0] : (const void *)pl.tw_f[0]);
-    const TC *twnx = (const TC *)(dbl ? (const void *)pl.twn_d : (const void *)pl.twn_f);
-    const TC *twy = (const TC *)(dbl ? (const void *)pl.tw_d[1] : (const void *)pl.tw_f[1]);
-    const TC *twz = (const TC *)(dbl ? (const void *)pl.tw_d[2] : (const void *)pl.tw_f[2]);
-    PoissonMul<TC> pm{pl.kx2, pl.ky2, pl.kz2, nxh, pl.zero_policy};
+    const Tw<TC> tw(pl);
+    const SpecMul pm{pl.kx2, pl.ky2, pl.kz2, pl.zero_policy, 1.0};
     const long nlines = (long)ny * nz;
     const TC scale = (TC)(1.0 / ((double)L * ny * nz));
     // half spectrum (nxh, ny, nz) stored with row pitch nxp (a multiple of 16
     // complex values) so every tile row starts on a 128-byte boundary
     const int nxp = pl.nxp;
-    x_fft<TS, TC>(L, false, rhs, spec, nlines, twx, twnx, scale, nxp, st);
-    strided_fft<TS, TC, 0>(ny, spec, (long)nxh, (long)nxp * ny, (long)nxp, twy, pm, nz, st);
-    strided_fft<TS, TC, 2>(nz, spec, (long)nxh, (long)nxp, (long)nxp * ny, twz, pm, ny, st);
-    strided_fft<TS, TC, 1>(ny, spec, (long)nxh, (long)nxp * ny, (long)nxp, twy, pm, nz, st);
-    x_fft<TS, TC>(L, true, spec, f, nlines, twx, twnx, scale, nxp, st);
+    x_fft<TS, TC>(L, false, rhs, spec, nlines, tw.x, tw.xn, scale, nxp, 0, st);
+    strided_fft<TS, TC, 0>(ny, spec, spec, (long)nxh, (long)nxp * ny, (long)nxp, tw.y, pm, nz, st);
+    strided_fft<TS, TC, 2>(nz, spec, spec, (long)nxh, (long)nxp, (long)nxp * ny, tw.z, pm, ny, st);
+    strided_fft<TS, TC, 1>(ny, spec, spec, (long)nxh, (long)nxp * ny, (long)nxp, tw.y, pm, nz, st);
+    x_fft<TS, TC>(L, true, spec, f, nlines, tw.x, tw.xn, scale, nxp, 0, st);
     return hipGetLastError();
 }
 
@@ -439,6 +470,78 @@ hipError_t run_poisson(const PoissonPlanDev &pl, int mode, const void *rhs, void
     case 0: return run_poisson_t<double, double>(pl, (const double *)rhs, (double *)f, (double *)spec, st);
     case 1: return run_poisson_t<float, float>(pl, (const float *)rhs, (float *)f, (float *)spec, st);
     case 2: return run_poisson_t<float, double>(pl, (const float *)rhs, (float *)f, (float *)spec, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+// 2D transforms of the (x, y) planes of (nx, ny, nplanes) data; the half
+// spectrum is (nx/2+1, ny, nplanes) without padding.
+template <typename TS, typename TC>
+static hipError_t run_fft2_t(const PoissonPlanDev &pl, const TS *in, TS *out, int nplanes, hipStream_t st)
+{
+    const int nx = pl.nx, ny = pl.ny, nxh = nx / 2 + 1, L = nx / 2;
+    const Tw<TC> tw(pl);
+    const SpecMul pm{};
+    x_fft<TS, TC>(L, false, in, out, (long)ny * nplanes, tw.x, tw.xn, (TC)1, nxh, 0, st);
+    strided_fft<TS, TC, 0>(ny, out, out, (long)nxh, (long)nxh * ny, (long)nxh, tw.y, pm, nplanes, st);
+    return hipGetLastError();
+}
+
+template <typename TS, typename TC>
+static hipError_t run_ifft2_t(const PoissonPlanDev &pl, const TS *in, TS *work, TS *out, int nplanes, int oddball,
+                              hipStream_t st)
+{
+    const int nx = pl.nx, ny = pl.ny, nxh = nx / 2 + 1, L = nx / 2;
+    const Tw<TC> tw(pl);
+    const SpecMul pm{};
+    // the c2r pass returns L times the inverse transform: scale to 1/(nx ny)
+    const TC scale = (TC)(1.0 / ((double)L * ny));
+    strided_fft<TS, TC, 1>(ny, in, work, (long)nxh, (long)nxh * ny, (long)nxh, tw.y, pm, nplanes, st);
+    x_fft<TS, TC>(L, true, work, out, (long)ny * nplanes, tw.x, tw.xn, scale, nxh, oddball, st);
+    return hipGetLastError();
+}
+
+// forward z transform -> multiply -> inverse z transform of (nxh, ny, nz) complex data
+template <typename TS, typename TC>
+static hipError_t run_zmul_t(const PoissonPlanDev &pl, const TS *in, TS *out, const SpecMul &m, hipStream_t st)
+{
+    const int ny = pl.ny, nz = pl.nz, nxh = pl.nx / 2 + 1;
+    const Tw<TC> tw(pl);
+    strided_fft<TS, TC, 2>(nz, in, out, (long)nxh, (long)nxh, (long)nxh * ny, tw.z, m, ny, st);
+    return hipGetLastError();
+}
+
+hipError_t run_fft2(const PoissonPlanDev &pl, int mode, const void *in, void *out, int nplanes, hipStream_t st)
+{
+    switch (mode) {
+    case 0: return run_fft2_t<double, double>(pl, (const double *)in, (double *)out, nplanes, st);
+    case 1: return run_fft2_t<float, float>(pl, (const float *)in, (float *)out, nplanes, st);
+    case 2: return run_fft2_t<float, double>(pl, (const float *)in, (float *)out, nplanes, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t run_ifft2(const PoissonPlanDev &pl, int mode, const void *in, void *work, void *out, int nplanes,
+                     int oddball, hipStream_t st)
+{
+    switch (mode) {
+    case 0:
+        return run_ifft2_t<double, double>(pl, (const double *)in, (double *)work, (double *)out, nplanes, oddball,
+                                           st);
+    case 1:
+        return run_ifft2_t<float, float>(pl, (const float *)in, (float *)work, (float *)out, nplanes, oddball, st);
+    case 2:
+        return run_ifft2_t<float, double>(pl, (const float *)in, (float *)work, (float *)out, nplanes, oddball, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t run_zmul(const PoissonPlanDev &pl, int mode, const void *in, void *out, const SpecMul &m, hipStream_t st)
+{
+    switch (mode) {
+    case 0: return run_zmul_t<double, double>(pl, (const double *)in, (double *)out, m, st);
+    case 1: return run_zmul_t<float, float>(pl, (const float *)in, (float *)out, m, st);
+    case 2: return run_zmul_t<float, double>(pl, (const float *)in, (float *)out, m, st);
     }
     return hipErrorInvalidValue;
 }

@@ -5,11 +5,16 @@
 
 namespace pdo {
 
-// squared modified wavenumbers (FP64) of the half spectrum (nx/2+1, ny, nz)
-template <typename TC> struct PoissonMul {
-    const double *kx2, *ky2, *kz2;
-    int nxh;
-    int zero_policy;
+// Factor applied between the forward and inverse transforms of a z line
+// (kx, ky, kz):
+//   kind 0: -1/(ax[kx]+ay[ky]+az[kz]+1e-20), zero at (0,0,0)    (PoissonPeriodic)
+//   kind 1: -1/(ax+ay+az), zero where ax+ay+az <= 1e-14          (PadePoisson)
+//   kind 2: ax[kx]*ay[ky]*az[kz]                                   (dealias mask)
+// then times `scale`.
+struct SpecMul {
+    const double *ax, *ay, *az;
+    int kind;
+    double scale;
 };
 
 struct PoissonPlanDev {
@@ -26,5 +31,16 @@ struct PoissonPlanDev {
 // mode 0: FP64, 1: FP32 storage and arithmetic, 2: FP32 storage, FP64 arithmetic.
 // spec: work buffer of nxp*ny*nz complex values of the storage type.
 hipError_t run_poisson(const PoissonPlanDev &pl, int mode, const void *rhs, void *f, void *spec, hipStream_t st);
+
+// 2D (x r2c, y c2c) forward transform of each of `nplanes` (nx, ny) planes into
+// an unpadded (nx/2+1, ny, nplanes) spectrum; unnormalized.
+hipError_t run_fft2(const PoissonPlanDev &pl, int mode, const void *in, void *out, int nplanes, hipStream_t st);
+// inverse of run_fft2 scaled by 1/(nx ny); `in` is not modified (work holds
+// the y-inverse planes); oddball != 0 drops the kx = nx/2 modes.
+hipError_t run_ifft2(const PoissonPlanDev &pl, int mode, const void *in, void *work, void *out, int nplanes,
+                     int oddball, hipStream_t st);
+// z transform, multiply by m, inverse z transform of an unpadded
+// (nx/2+1, ny, nz) spectrum (in may equal out).
+hipError_t run_zmul(const PoissonPlanDev &pl, int mode, const void *in, void *out, const SpecMul &m, hipStream_t st);
 
 }  // namespace pdo

@@ -70,7 +70,14 @@ static int build_desc(const OpObject &o, int kind, int bc1, int bcn, SystemDesc 
     case SCHEME_CD06: return build_cd06(s, kind, o.n, o.dx, o.periodic, o.bc1, o.bcn);
     case SCHEME_CD10: return build_cd10(s, kind, o.n, o.dx, o.periodic, o.periodic ? 0 : bc1, o.periodic ? 0 : bcn);
     case SCHEME_CF90: return build_cf90(s, o.n, o.periodic, o.periodic ? 0 : bc1, o.periodic ? 0 : bcn);
-    case SCHEME_CD06STAGG: return build_stagg_periodic(s, kind, o.n, o.dx);
+    case SCHEME_CD06STAGG:
+        if (o.periodic) {
+            // the collocated first derivatives exist only in the non-periodic
+            // object (cd06stagg.F90:883-925 read its Tri tables)
+            if (kind == KIND_STG_D1_C2C || kind == KIND_STG_D1_E2E) return PDO_ERR_UNSUP;
+            return build_stagg_periodic(s, kind, o.n, o.dx);
+        }
+        return build_stagg_nonperiodic(s, kind, o.n, o.dx, o.flags);
     }
     return PDO_ERR_ARG;
 }
@@ -207,6 +214,10 @@ int pdo_device_count(int *count)
     int c = 0;
     hipError_t e = hipGetDeviceCount(&c);
     if (count) *count = (e == hipSuccess) ? c : 0;
+    if (e == hipErrorNoDevice || (e == hipSuccess && c == 0)) {
+        set_error("no HIP device");
+        return PDO_ERR_NOGPU;
+    }
     if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
     return 0;
 }
@@ -391,6 +402,22 @@ int pdo_cd06stagg_init_periodic(pdo_op *h, int n, double dz)
     return make_op(h, SCHEME_CD06STAGG, n, dz, 1, 0, 0);
 }
 
+int pdo_cd06stagg_init(pdo_op *h, int n, double dz, int isTopEven, int isBotEven, int isTopSided, int isBotSided)
+{
+    if (n <= 4) {
+        set_error("CD06_stagg requires at least 4 points");  // cd06stagg.F90:216-218
+        return 21;
+    }
+    if (n < 8) {
+        set_error("non-periodic cd06stagg needs n >= 8 here (closure rows span 8 points)");
+        return PDO_ERR_UNSUP;
+    }
+    int st = make_op(h, SCHEME_CD06STAGG, n, dz, 0, 0, 0);
+    if (st) return st;
+    (*h)->flags = (isTopEven ? 1 : 0) | (isBotEven ? 2 : 0) | (isTopSided ? 4 : 0) | (isBotSided ? 8 : 0);
+    return 0;
+}
+
 #define PDO_STG(NAME, KIND)                                                                                   \
     int NAME(pdo_op h, int prec, const void *fi, void *fo, int n1, int n2, int is_complex, void *stream)    \
     {                                                                                                         \
@@ -403,6 +430,8 @@ PDO_STG(pdo_cd06stagg_d2dz2_C2C, KIND_STG_D2_C2C)
 PDO_STG(pdo_cd06stagg_d2dz2_E2E, KIND_STG_D2_E2E)
 PDO_STG(pdo_cd06stagg_interpz_E2C, KIND_STG_IN_E2C)
 PDO_STG(pdo_cd06stagg_interpz_C2E, KIND_STG_IN_C2E)
+PDO_STG(pdo_cd06stagg_ddz_C2C, KIND_STG_D1_C2C)
+PDO_STG(pdo_cd06stagg_ddz_E2E, KIND_STG_D1_E2E)
 #undef PDO_STG
 
 int pdo_op_destroy(pdo_op h)

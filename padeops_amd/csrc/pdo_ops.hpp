@@ -42,6 +42,7 @@ struct OpObject {
     double dx = 1.0;
     bool periodic = true;
     int bc1 = 0, bcn = 0;
+    int flags = 0;  // cd06stagg non-periodic: bit 0 isTopEven, 1 isBotEven, 2 isTopSided, 3 isBotSided
     std::map<int, std::unique_ptr<Variant>> variants;
     std::mutex mu;
     // build (or fetch) a variant; returns status (0 ok)

@@ -13,7 +13,7 @@ namespace pdo {
 
 // wavenumbers of PoissonPeriodic.F90:226-261 (GetWaveNums + ifftshift):
 // k = (0, 1, .., n/2-1, -n/2, .., -1) * 2 pi / (n dx)
-static std::vector<double> wavenums(int n, double dx)
+std::vector<double> wavenums(int n, double dx)
 {
     const int dummy = n - (n % 2);
     std::vector<double> t(n), k(n);
@@ -53,7 +53,7 @@ double modified_k(int kind, double k, double dx)
     }
 }
 
-template <typename T> static int upload(const std::vector<T> &h, T **d)
+template <typename T> int upload_vec(const std::vector<T> &h, T **d)
 {
     hipError_t e = hipMalloc(d, h.size() * sizeof(T));
     if (e != hipSuccess) return hip_fail(e, "hipMalloc");
@@ -61,7 +61,10 @@ template <typename T> static int upload(const std::vector<T> &h, T **d)
     if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
     return 0;
 }
+template int upload_vec<double>(const std::vector<double> &, double **);
+template int upload_vec<float>(const std::vector<float> &, float **);
 
+// exp(-2 pi i m / L), m < L, in FP64 and FP32 (computed in long double)
 static void twiddles(int L, std::vector<double> &d, std::vector<float> &f)
 {
     d.resize(2 * L);
@@ -75,7 +78,49 @@ static void twiddles(int L, std::vector<double> &d, std::vector<float> &f)
     }
 }
 
-static bool pow2(int n) { return n >= 4 && n <= 1024 && (n & (n - 1)) == 0; }
+bool fft_pow2(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+
+int fft_tables(PoissonPlanDev &dev, int nx, int ny, int nz, std::vector<void *> &allocs)
+{
+    int st;
+    const int nxh = nx / 2 + 1;
+    const int Ls[3] = {nx / 2, ny, nz};
+    for (int d = 0; d < 3; d++) {
+        if (Ls[d] <= 0) continue;
+        std::vector<double> td; std::vector<float> tf;
+        twiddles(Ls[d], td, tf);
+        double *pd; float *pf;
+        if ((st = upload_vec(td, &pd))) return st;
+        allocs.push_back(pd);
+        if ((st = upload_vec(tf, &pf))) return st;
+        allocs.push_back(pf);
+        dev.tw_d[d] = pd; dev.tw_f[d] = pf;
+    }
+    // exp(-2 pi i k / nx), k = 0..nx/2: the real-transform split
+    std::vector<double> td(2 * nxh); std::vector<float> tf(2 * nxh);
+    for (int k = 0; k < nxh; k++) {
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * k / nx;
+        td[2 * k] = (double)cosl(a); td[2 * k + 1] = (double)sinl(a);
+        tf[2 * k] = (float)cosl(a); tf[2 * k + 1] = (float)sinl(a);
+    }
+    double *pd; float *pf;
+    if ((st = upload_vec(td, &pd))) return st;
+    allocs.push_back(pd);
+    if ((st = upload_vec(tf, &pf))) return st;
+    allocs.push_back(pf);
+    dev.twn_d = pd; dev.twn_f = pf;
+    return 0;
+}
+
+int fft_mode(int prec)
+{
+    switch (prec) {
+    case PDO_F64: return 0;
+    case PDO_F32_REFINE: case PDO_F32: return 1;
+    case PDO_F32_STORE: return 2;
+    }
+    return -1;
+}
 
 PoissonObject::~PoissonObject()
 {
@@ -85,7 +130,7 @@ PoissonObject::~PoissonObject()
 
 int PoissonObject::init(int nx_, int ny_, int nz_, double dx, double dy, double dz, int mx, int my, int mz, int zp)
 {
-    if (!pow2(ny_) || !pow2(nz_) || !pow2(nx_) || nx_ < 16 || ny_ < 8 || nz_ < 8) {
+    if (!fft_pow2(nx_, 16, 1024) || !fft_pow2(ny_, 8, 1024) || !fft_pow2(nz_, 8, 1024)) {
         set_error("Poisson solver sizes must be powers of two in [16,1024] (x) / [8,1024] (y, z)");
         return PDO_ERR_ARG;
     }
@@ -104,48 +149,20 @@ int PoissonObject::init(int nx_, int ny_, int nz_, double dx, double dy, double 
     };
     double *p;
     int st;
-    if ((st = upload(k2(nx, dx, mx, nxh), &p))) return st;
+    if ((st = upload_vec(k2(nx, dx, mx, nxh), &p))) return st;
     allocs.push_back(p); dev.kx2 = p;
-    if ((st = upload(k2(ny, dy, my, ny), &p))) return st;
+    if ((st = upload_vec(k2(ny, dy, my, ny), &p))) return st;
     allocs.push_back(p); dev.ky2 = p;
-    if ((st = upload(k2(nz, dz, mz, nz), &p))) return st;
+    if ((st = upload_vec(k2(nz, dz, mz, nz), &p))) return st;
     allocs.push_back(p); dev.kz2 = p;
-    const int Ls[3] = {nx / 2, ny, nz};
-    for (int d = 0; d < 3; d++) {
-        std::vector<double> td; std::vector<float> tf;
-        twiddles(Ls[d], td, tf);
-        double *pd; float *pf;
-        if ((st = upload(td, &pd))) return st;
-        if ((st = upload(tf, &pf))) return st;
-        allocs.push_back(pd); allocs.push_back(pf);
-        dev.tw_d[d] = pd; dev.tw_f[d] = pf;
-    }
-    {
-        std::vector<double> td(2 * nxh); std::vector<float> tf(2 * nxh);
-        for (int k = 0; k < nxh; k++) {
-            const long double a = -2.0L * 3.14159265358979323846264338327950288L * k / nx;
-            td[2 * k] = (double)cosl(a); td[2 * k + 1] = (double)sinl(a);
-            tf[2 * k] = (float)cosl(a); tf[2 * k + 1] = (float)sinl(a);
-        }
-        double *pd; float *pf;
-        if ((st = upload(td, &pd))) return st;
-        if ((st = upload(tf, &pf))) return st;
-        allocs.push_back(pd); allocs.push_back(pf);
-        dev.twn_d = pd; dev.twn_f = pf;
-    }
-    return 0;
+    return fft_tables(dev, nx, ny, nz, allocs);
 }
 
 int PoissonObject::solve(int prec, const void *rhs, void *f, hipStream_t st)
 {
     if (!rhs || !f) { set_error("null field"); return PDO_ERR_ARG; }
-    int mode;
-    switch (prec) {
-    case PDO_F64: mode = 0; break;
-    case PDO_F32_REFINE: case PDO_F32: mode = 1; break;
-    case PDO_F32_STORE: mode = 2; break;
-    default: set_error("unknown precision mode"); return PDO_ERR_ARG;
-    }
+    const int mode = fft_mode(prec);
+    if (mode < 0) { set_error("unknown precision mode"); return PDO_ERR_ARG; }
     const size_t need = (size_t)dev.nxp * ny * nz * 2 * (mode == 0 ? 8 : 4);
     if (spec_bytes < need) {
         if (spec) (void)hipFree(spec);

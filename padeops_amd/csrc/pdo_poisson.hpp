@@ -10,6 +10,14 @@
 namespace pdo {
 
 double modified_k(int kind, double k, double dx);
+// GetWaveNums + ifftshift: (0, 1, .., n/2-1, -n/2, .., -1) * 2 pi / (n dx)
+std::vector<double> wavenums(int n, double dx);
+template <typename T> int upload_vec(const std::vector<T> &h, T **d);
+bool fft_pow2(int n, int lo, int hi);
+// twiddle tables of the x (length nx/2 plus the split), y and z (nz > 0) transforms
+int fft_tables(PoissonPlanDev &dev, int nx, int ny, int nz, std::vector<void *> &allocs);
+// storage/arithmetic mode of the FFT kernels for a precision mode (-1: unknown)
+int fft_mode(int prec);
 
 struct PoissonObject {
     int nx = 0, ny = 0, nz = 0;

@@ -356,6 +356,218 @@ int build_stagg_periodic(SystemDesc &s, int kind, int n, double dz)
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// cd06stagg, non-periodic (cd06stagg.F90:197-232, STAGG_CD06_files/*.F90)
+//
+// The even / odd boundary variants are the periodic stencils applied to the
+// even / odd reflection of the field about the wall: a cell grid (n points)
+// mirrors about the half-cell walls, an edge grid (n+1 points) about its end
+// points.  First derivatives flip the parity between input and output,
+// second derivatives and interpolation keep it.  The ghost values are folded
+// into explicit closure rows (RHS) and into the tridiagonal rows (LHS).  The
+// "sided" variants replace the end rows by the one-sided closures of the
+// reference (ComputeTri_allRoutines.F90, D1RHS_*_common.F90,
+// InterpRHS_*_common.F90); D2 and the E2E first derivative have none.
+// ---------------------------------------------------------------------------
+namespace {
+struct Reflect {
+    bool edge;   // edge grid (n+1 points) or cell grid (n points)
+    int n;       // cells
+    int pb, pt;  // parity (+1 even, -1 odd) at the bottom and top wall
+    // map index r (any integer near the range) to (index in range, sign)
+    void map(int r, int &j, double &sg) const
+    {
+        sg = 1.0;
+        const int last = edge ? n : n - 1;
+        for (int it = 0; it < 4 && (r < 0 || r > last); it++) {
+            if (r < 0) { r = edge ? -r : -1 - r; sg *= pb; }
+            else { r = edge ? 2 * n - r : 2 * n - 1 - r; sg *= pt; }
+        }
+        j = r;
+    }
+};
+}  // namespace
+
+int build_stagg_nonperiodic(SystemDesc &s, int kind, int n, double dz, int flags)
+{
+    if (n <= 4) return 21;  // cd06stagg.F90:216-218
+    const bool te = flags & 1, be = flags & 2, ts = flags & 4, bs = flags & 8;
+    s = SystemDesc();
+    s.periodic = false;
+    s.K = 1;
+    s.RB = 2;
+    const double odz = 1.0 / dz;
+    const double aD1 = (63.0 / 62.0) * odz, bD1 = ((17.0 / 62.0) / 3.0) * odz;
+    const double aD2 = (12.0 / 11.0) * odz * odz, bD2 = ((3.0 / 11.0) / 4.0) * odz * odz;
+    const double aI = 1.5 * 0.5, bI = 0.1 * 0.5;
+    const double aC = ((14.0 / 9.0) / 2.0) * odz, bC = ((1.0 / 9.0) / 4.0) * odz;
+    bool in_edge, out_edge, flip, has_sided;
+    double al;
+    switch (kind) {
+    case KIND_STG_D1_E2C:
+        s.family = FAM_STG_E2C_A; s.c[1] = aD1; s.c[2] = bD1; al = 9.0 / 62.0;
+        in_edge = true; out_edge = false; flip = true; has_sided = true; break;
+    case KIND_STG_D1_C2E:
+        s.family = FAM_STG_C2E_A; s.c[1] = aD1; s.c[2] = bD1; al = 9.0 / 62.0;
+        in_edge = false; out_edge = true; flip = true; has_sided = true; break;
+    case KIND_STG_IN_E2C:
+        s.family = FAM_STG_E2C_S; s.c[1] = aI; s.c[2] = bI; al = 3.0 / 10.0;
+        in_edge = true; out_edge = false; flip = false; has_sided = true; break;
+    case KIND_STG_IN_C2E:
+        s.family = FAM_STG_C2E_S; s.c[1] = aI; s.c[2] = bI; al = 3.0 / 10.0;
+        in_edge = false; out_edge = true; flip = false; has_sided = true; break;
+    case KIND_STG_D2_C2C:
+        s.family = FAM_SYM2C; s.c[0] = -2 * (aD2 + bD2); s.c[1] = aD2; s.c[2] = bD2; al = 2.0 / 11.0;
+        in_edge = false; out_edge = false; flip = false; has_sided = false; break;
+    case KIND_STG_D2_E2E:
+        s.family = FAM_SYM2C; s.c[0] = -2 * (aD2 + bD2); s.c[1] = aD2; s.c[2] = bD2; al = 2.0 / 11.0;
+        in_edge = true; out_edge = true; flip = false; has_sided = false; break;
+    case KIND_STG_D1_C2C:
+        s.family = FAM_CD06_D1; s.c[1] = aC; s.c[2] = bC; al = 1.0 / 3.0;
+        in_edge = false; out_edge = false; flip = true; has_sided = true; break;
+    case KIND_STG_D1_E2E:
+        s.family = FAM_CD06_D1; s.c[1] = aC; s.c[2] = bC; al = 1.0 / 3.0;
+        in_edge = true; out_edge = true; flip = true; has_sided = false; break;
+    default: return -1;
+    }
+    s.n_in = in_edge ? n + 1 : n;
+    s.n_sys = s.n_out = out_edge ? n + 1 : n;
+    const int N = s.n_sys, NI = s.n_in;
+    set_band(s, 0, al);
+    const Reflect rin{in_edge, n, be ? 1 : -1, te ? 1 : -1};
+    const int ob = flip ? -1 : 1;
+    const Reflect rout{out_edge, n, ob * rin.pb, ob * rin.pt};
+    const FamShape fs = family_shape(s.family);
+
+    // the interior stencil of row i, folded into range
+    auto folded_row = [&](int i) {
+        std::map<int, double> row;
+        auto add = [&](int r, double w) {
+            int j; double sg;
+            rin.map(r, j, sg);
+            row[j] += sg * w;
+        };
+        if (fs.center) add(i, s.c[0]);
+        for (int q = 1; q <= fs.R; q++) {
+            int hi, lo;
+            pair_offsets(fs, q, hi, lo);
+            add(i + hi, s.c[q]);
+            add(i + lo, fs.sigma * s.c[q]);
+        }
+        return row;
+    };
+    auto reaches_out = [&](int i) {
+        for (int q = 1; q <= fs.R; q++) {
+            int hi, lo;
+            pair_offsets(fs, q, hi, lo);
+            if (i + lo < 0 || i + hi >= NI) return true;
+        }
+        return false;
+    };
+    auto set_special = [&](int i, const std::map<int, double> &row) {
+        std::vector<std::pair<int, double>> v;
+        for (auto &kv : row) if (kv.second != 0.0) v.push_back({kv.first, kv.second});
+        s.special[i] = v;
+    };
+    // even / odd closures (RHS and LHS folding)
+    for (int i = 0; i < N; i++) {
+        if (reaches_out(i)) set_special(i, folded_row(i));
+        for (int o = -1; o <= 1; o += 2) {
+            const int jj = i + o;
+            if (jj >= 0 && jj < N) continue;
+            int j; double sg;
+            rout.map(jj, j, sg);
+            const double w = s.lhs[i][o + 2];
+            s.lhs[i][o + 2] = 0.0;
+            s.lhs[i][j - i + 2] += sg * w;
+        }
+    }
+    // one-sided closures
+    if (has_sided && (bs || ts)) {
+        const double w0s = 223.0 / 186.0, w1s = 61.0 / 62.0;
+        typedef std::vector<std::pair<int, double>> Row;
+        auto put = [&](bool bottom, int k, const Row &rhs, double lo, double dg, double up) {
+            // bottom: row k from the start, columns as given; top: mirrored
+            // row and columns, LHS neighbours swapped (the reference's
+            // top rows are the mirror images of the bottom ones)
+            Row r;
+            for (auto &cw : rhs) r.push_back({bottom ? cw.first : NI - 1 - cw.first, cw.second});
+            const int i = bottom ? k : N - 1 - k;
+            s.special[i] = r;
+            s.lhs[i] = {0.0, bottom ? lo : up, dg, bottom ? up : lo, 0.0};
+        };
+        for (int side = 0; side < 2; side++) {
+            const bool bottom = side == 0;
+            if (bottom ? !bs : !ts) continue;
+            const double sgn = bottom ? 1.0 : -1.0;  // first derivatives change sign under mirroring
+            switch (kind) {
+            case KIND_STG_D1_E2C: {  // D1RHS_E2C_common.F90, ComputeTriD1_E2C
+                const double alpha1 = 37.0 / 183.0, alpha0 = -1.0;
+                const double a0 = (1.0 / 24.0) * (alpha0 - 23.0), b0 = (1.0 / 8.0) * (-9.0 * alpha0 + 7.0);
+                const double c0 = (1.0 / 8.0) * (9.0 * alpha0 + 1.0), d0 = -(1.0 / 24.0) * (alpha0 + 1.0);
+                const double a1 = (3.0 / 8.0) * (3.0 - 2.0 * alpha1), b1 = (1.0 / 8.0) * (-1.0 + 22.0 * alpha1);
+                const double f0 = sgn * w0s * odz, f1 = sgn * w1s * odz;
+                put(bottom, 0, {{0, f0 * a0}, {1, f0 * b0}, {2, f0 * c0}, {3, f0 * d0}}, 0.0, w0s, w0s * alpha0);
+                put(bottom, 1, {{0, f1 * (-b1 / 3.0)}, {1, f1 * (-a1)}, {2, f1 * a1}, {3, f1 * (b1 / 3.0)}},
+                    w1s * alpha1, w1s, w1s * alpha1);
+                break;
+            }
+            case KIND_STG_D1_C2E: {  // D1RHS_C2E_common.F90, ComputeTriD1_C2E
+                const double a0 = -71.0 / 24.0, b0 = 47.0 / 8.0, c0 = -31.0 / 8.0, d0 = 23.0 / 24.0;
+                const double a1 = 12.0 / 11.0;
+                const double f0 = sgn * odz, f1 = sgn * a1 * odz;
+                put(bottom, 0, {{0, f0 * a0}, {1, f0 * b0}, {2, f0 * c0}, {3, f0 * d0}}, 0.0, 1.0, 0.0);
+                put(bottom, 1, {{0, -f1}, {1, f1}}, 1.0 / 22.0, 1.0, 1.0 / 22.0);
+                break;
+            }
+            case KIND_STG_D1_C2C: {  // D1RHS_C2C_common.F90, ComputeTriD1_C2C (Lele closures)
+                const double ah = 1.0 / 3.0, p = 17.0 / 6.0, q = 1.5, r = 1.5, sl = -1.0 / 6.0, aLOW = 3.0;
+                const double q_p = 0.75, alpha_p = 0.25;
+                const double alpha_pp = ((40 * ah - 1) * q + 7 * (4 * ah - 1) * sl) /
+                                        (16 * (ah + 2) * q + 8 * (1 - 4 * ah) * sl);
+                const double q_pp = (1.0 / 3.0) * (alpha_pp + 2), r_pp = (1.0 / 12.0) * (4 * alpha_pp - 1);
+                const double w1 = (2 * ah + 1) / (2 * (q + sl));
+                const double w2 = ((8 * ah + 7) * q - 6 * (2 * ah + 1) * r + (8 * ah + 7) * sl) / (9 * (q + sl));
+                const double w3 = (4 * (ah + 2) * q + 2 * (1 - 4 * ah) * sl) / (9 * (q + sl));
+                const double g = sgn * odz;
+                put(bottom, 0, {{0, g * w1 * (-p)}, {1, g * w1 * q}, {2, g * w1 * r}, {3, g * w1 * sl}}, 0.0, w1,
+                    w1 * aLOW);
+                put(bottom, 1, {{0, -g * w2 * q_p}, {2, g * w2 * q_p}}, w2 * alpha_p, w2, w2 * alpha_p);
+                put(bottom, 2, {{0, -g * w3 * r_pp}, {1, -g * w3 * q_pp}, {3, g * w3 * q_pp}, {4, g * w3 * r_pp}},
+                    w3 * alpha_pp, w3, w3 * alpha_pp);
+                break;
+            }
+            case KIND_STG_IN_E2C: {  // InterpRHS_E2C_common.F90, ComputeTriInterp_E2C
+                const double alpha0 = 1.0;
+                const double a0 = (1.0 / 16.0) * (5.0 - alpha0), b0 = (1.0 / 16.0) * (9.0 * alpha0 + 15.0);
+                const double c0 = (1.0 / 16.0) * (9.0 * alpha0 - 5.0), d0 = (1.0 / 16.0) * (1.0 - alpha0);
+                put(bottom, 0, {{0, a0}, {1, b0}, {2, c0}, {3, d0}}, 0.0, 1.0, alpha0);
+                break;
+            }
+            case KIND_STG_IN_C2E: {  // InterpRHS_C2E_common.F90, ComputeTriInterp_C2E
+                const double a0 = 15.0 / 8.0, b0 = -5.0 / 4.0, c0 = 3.0 / 8.0;
+                const double alpha1 = 1.0 / 6.0, a1 = (1.0 / 8.0) * (9.0 + 10.0 * alpha1);
+                put(bottom, 0, {{0, a0}, {1, b0}, {2, c0}}, 0.0, 1.0, 0.0);
+                put(bottom, 1, {{0, a1 / 2.0}, {1, a1 / 2.0}}, alpha1, 1.0, alpha1);
+                break;
+            }
+            }
+        }
+    }
+    // every row up to the last closure row of a side is explicit (the kernels
+    // read those rows from the closure table)
+    int lastL = -1, firstR = N;
+    for (auto &kv : s.special) {
+        if (kv.first < N / 2) lastL = std::max(lastL, kv.first);
+        else firstR = std::min(firstR, kv.first);
+    }
+    for (int i = 0; i <= lastL; i++)
+        if (!s.special.count(i)) set_special(i, folded_row(i));
+    for (int i = firstR; i < N; i++)
+        if (!s.special.count(i)) set_special(i, folded_row(i));
+    return 0;
+}
+
 // number of input points read directly before wrapping (see kernels)
 static int n_read(const SystemDesc &s)
 {

@@ -36,7 +36,7 @@ enum Family : int {
 enum OpKind : int {
     KIND_D1 = 0, KIND_D2 = 1, KIND_FILTER = 2,
     KIND_STG_D1_E2C = 10, KIND_STG_D1_C2E = 11, KIND_STG_D2_C2C = 12, KIND_STG_D2_E2E = 13,
-    KIND_STG_IN_E2C = 14, KIND_STG_IN_C2E = 15
+    KIND_STG_IN_E2C = 14, KIND_STG_IN_C2E = 15, KIND_STG_D1_C2C = 16, KIND_STG_D1_E2E = 17
 };
 
 enum Scheme : int { SCHEME_CD06 = 0, SCHEME_CD10 = 1, SCHEME_CF90 = 2, SCHEME_CD06STAGG = 3 };
@@ -65,6 +65,8 @@ int build_cd06(SystemDesc &s, int kind, int n, double dx, bool periodic, int bc1
 int build_cd10(SystemDesc &s, int kind, int n, double dx, bool periodic, int bc1, int bcn);
 int build_cf90(SystemDesc &s, int n, bool periodic, int bc1, int bcn);
 int build_stagg_periodic(SystemDesc &s, int kind, int n, double dz);
+// flags: bit 0 isTopEven, 1 isBotEven, 2 isTopSided, 3 isBotSided
+int build_stagg_nonperiodic(SystemDesc &s, int kind, int n, double dz, int flags);
 
 // Evaluate the RHS stencil row i on input f (host, for dense export/tests).
 double rhs_row(const SystemDesc &s, int i, const double *f);
