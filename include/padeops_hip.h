@@ -64,6 +64,8 @@ typedef struct pdo_poisson_s *pdo_poisson;
 typedef struct pdo_rhs_s *pdo_rhs;
 typedef struct pdo_spectral_s *pdo_spectral;
 typedef struct pdo_padepoisson_s *pdo_padepoisson;
+typedef struct pdo_pade6stagg_s *pdo_pade6stagg;
+typedef struct pdo_duidxj_s *pdo_duidxj;
 
 /* ---- library ------------------------------------------------------------ */
 const char *pdo_last_error(void);
@@ -182,6 +184,20 @@ int pdo_poisson_periodic_init(pdo_poisson *h, int nx, int ny, int nz, double dx,
 int pdo_poisson_periodic_solve(pdo_poisson h, int prec, const void *rhs, void *f, void *stream);
 int pdo_poisson_destroy(pdo_poisson h);
 
+/* ---- Pade6stagg: igrid's vertical staggered derivative object -------------
+ * replaces type(Pade6stagg) of src/incompressible/PadeDerOps.F90 with scheme
+ * cd06: init :57-124 (periodic: one periodic cd06stagg; otherwise the nine
+ * boundary objects derOO ... derSS), and the generics ddz_C2E / ddz_E2C /
+ * d2dz2_C2C / d2dz2_E2E / interpz_C2E / interpz_E2C :146-990 selected by
+ * kind = 10..15 (as pdo_op_dense) and (bot, top) in {-1 odd, 0 one-sided,
+ * 1 even}; pairs without an object write zeros like the reference.
+ * getModifiedWavenumbers :997-1075: scheme 0 fd02, 1 cd06, 2 Fourier. */
+int pdo_pade6stagg_init(pdo_pade6stagg *h, int nz, double dz, int periodic);
+int pdo_pade6stagg_apply(pdo_pade6stagg h, int kind, int bot, int top, int prec, const void *in, void *out, int n1,
+                         int n2, int is_complex, void *stream);
+int pdo_pade6stagg_modk(int scheme, int n, double dz, const double *k, double *kp);
+int pdo_pade6stagg_destroy(pdo_pade6stagg h);
+
 /* ---- spectral: 2D transforms of the igrid solvers --------------------------
  * replaces type(spectral) of src/incompressible/spectral.F90 as igrid uses it
  * (dimTransform = 2, x input pencil, igrid.F90:489-493):
@@ -229,6 +245,29 @@ int pdo_padepoisson_projection(pdo_padepoisson h, int prec, void *uhat, void *vh
 int pdo_padepoisson_divergence(pdo_padepoisson h, int prec, const void *uhat, const void *vhat, const void *what,
                                void *divergence, void *stream);
 int pdo_padepoisson_destroy(pdo_padepoisson h);
+
+/* ---- igrid%compute_duidxj --------------------------------------------------
+ * replaces igrid.F90:2553-2683 for one GPU (the y <-> z transposes of the
+ * reference are identities there).  Inputs are igrid's spectral velocity:
+ * uhat, vhat, whatC on cells (nx/2+1, ny, nz) and what, uEhat, vEhat on edges
+ * (nx/2+1, ny, nz+1).  Outputs, in igrid's component order 1 dudx, 2 dudy,
+ * 3 dudz, 4 dvdx, 5 dvdy, 6 dvdz, 7 dwdx, 8 dwdy, 9 dwdz:
+ *   duidxjC (nx, ny, nz, 9) and duidxjE (nx, ny, nz+1, 9) real,
+ *   duidxjChat / duidxjEhat their spectra (the reference fills them likewise),
+ *   d2wdz2hatE, d2udz2hatC, d2vdz2hatC (viscous only; NULL when inviscid).
+ * spC: spectral object of (nx, ny, nz); spE: of (nx, ny, nz+1); dz: the
+ * Pade6stagg object.  bc[12] = {uBC_bottom, uBC_top, vBC_bottom, vBC_top,
+ * wBC_bottom, wBC_top, dUdzBC_bottom, dUdzBC_top, dVdzBC_bottom, dVdzBC_top,
+ * dWdzBC_bottom, dWdzBC_top} (ignored by a periodic Pade6stagg). */
+typedef struct {
+    const void *uhat, *vhat, *whatC, *what, *uEhat, *vEhat;
+    void *duidxjC, *duidxjE, *duidxjChat, *duidxjEhat;
+    void *d2wdz2hatE, *d2udz2hatC, *d2vdz2hatC;
+} pdo_duidxj_fields;
+int pdo_duidxj_init(pdo_duidxj *h, pdo_spectral spC, pdo_spectral spE, pdo_pade6stagg dz, const int *bc,
+                    int inviscid);
+int pdo_duidxj_compute(pdo_duidxj h, int prec, const pdo_duidxj_fields *fields, void *stream);
+int pdo_duidxj_destroy(pdo_duidxj h);
 
 /* ---- full RHS evaluation (the benchmarked composite) ----------------------
  * On a triply periodic n^3 grid of side L, for velocity (u, v, w):

@@ -12,7 +12,8 @@ brute-force DFT.
 """
 import numpy as np
 
-from . import STG_D1_C2E, STG_D1_E2C, apply, modk
+from . import (BOT_EVEN, BOT_SIDED, STG_D1_C2E, STG_D1_E2C, STG_D2_C2C, STG_D2_E2E, STG_IN_C2E, STG_IN_E2C,
+               TOP_EVEN, TOP_SIDED, apply, modk, stagg_np)
 
 
 def getwavenums(n, dx):
@@ -136,3 +137,66 @@ class PadePoissonPeriodic:
     def divergence(self, uhat, vhat, what):
         """DivergenceCheck steps 1-3: the real divergence field."""
         return self.sp.ifft(self._div(uhat, vhat, what))
+
+
+# Pade6stagg (PadeDerOps.F90:57-1008), scheme cd06: the boundary object of
+# each (bot, top) in {-1 odd, 0 one-sided, 1 even}; the object derXY is
+# initialised with X the top and Y the bottom kind, a one-sided side being
+# marked even unless the other side is odd.
+def pade6_flags(bot, top):
+    te = 1 if top == 1 else (0 if top == -1 else (0 if bot == -1 else 1))
+    be = 1 if bot == 1 else (0 if bot == -1 else (0 if top == -1 else 1))
+    return te * TOP_EVEN + be * BOT_EVEN + (TOP_SIDED if top == 0 else 0) + (BOT_SIDED if bot == 0 else 0)
+
+
+def pade6_apply(op, f, dz, nz, bot, top, periodic):
+    """A Pade6stagg generic on a complex field f(n1, n2, n_in)."""
+    f = np.asarray(f, dtype=np.complex128)
+    if periodic:
+        return _stagg(op, f, dz, nz)
+    if op in (STG_D2_C2C, STG_D2_E2E) and (bot == 0 or top == 0) or bot not in (-1, 0, 1) or top not in (-1, 0, 1):
+        nout = nz + 1 if op in (STG_D1_C2E, STG_IN_C2E, STG_D2_E2E) else nz
+        return np.zeros(f.shape[:2] + (nout,), complex)
+    fl = pade6_flags(bot, top)
+    return stagg_np(op, np.asfortranarray(f.real), dz, fl, nz) + 1j * stagg_np(op, np.asfortranarray(f.imag), dz,
+                                                                              fl, nz)
+
+
+def duidxj(spC, spE, dz, periodic, bc, inviscid, uhat, vhat, whatC, what, uEhat, vEhat):
+    """igrid%compute_duidxj (igrid.F90:2553-2683).  Returns a dict of the
+    outputs: duidxjC (nx,ny,nz,9), duidxjE (nx,ny,nz+1,9), their spectra and
+    d2wdz2hatE, d2udz2hatC, d2vdz2hatC (viscous)."""
+    nx, ny, nz = spC.shape
+    uB, uT, vB, vT, wB, wT, dUB, dUT, dVB, dVT, dWB, dWT = bc
+    RC = np.zeros((nx, ny, nz, 9))
+    RE = np.zeros((nx, ny, nz + 1, 9))
+    HC = np.zeros((spC.nxh, ny, nz, 9), complex)
+    HE = np.zeros((spC.nxh, ny, nz + 1, 9), complex)
+    out = {}
+    for k, c, e in ((1, uhat, uEhat), (2, uhat, uEhat), (4, vhat, vEhat), (5, vhat, vEhat), (7, whatC, what),
+                    (8, whatC, what)):
+        d = 1 if k in (1, 4, 7) else 2
+        HC[..., k - 1] = spC.mtimes_ik(c, d)
+        RC[..., k - 1] = spC.ifft(HC[..., k - 1])
+        HE[..., k - 1] = spE.mtimes_ik(e, d)
+        RE[..., k - 1] = spE.ifft(HE[..., k - 1])
+    P6 = lambda op, f, b, t: pade6_apply(op, f, dz, nz, b, t, periodic)  # noqa: E731
+    HC[..., 8] = P6(STG_D1_E2C, what, wB, wT)
+    RC[..., 8] = spC.ifft(HC[..., 8])
+    HE[..., 8] = P6(STG_IN_C2E, HC[..., 8], dWB, dWT)
+    RE[..., 8] = spE.ifft(HE[..., 8])
+    if not inviscid:
+        out["d2wdz2hatE"] = P6(STG_D2_E2E, what, wB, wT)
+    for k, c, b, t, db, dt, name in ((3, uhat, uB, uT, dUB, dUT, "d2udz2hatC"),
+                                     (6, vhat, vB, vT, dVB, dVT, "d2vdz2hatC")):
+        HE[..., k - 1] = P6(STG_D1_C2E, c, b, t)
+        RE[..., k - 1] = spE.ifft(HE[..., k - 1])
+        if not inviscid:
+            if t == 0 or b == 0:
+                out[name] = P6(STG_D1_E2C, P6(STG_D1_C2E, c, b, t), db, dt)
+            else:
+                out[name] = P6(STG_D2_C2C, c, b, t)
+        HC[..., k - 1] = P6(STG_IN_E2C, HE[..., k - 1], db, dt)
+        RC[..., k - 1] = spC.ifft(HC[..., k - 1])
+    out.update(duidxjC=RC, duidxjE=RE, duidxjChat=HC, duidxjEhat=HE)
+    return out

@@ -450,3 +450,107 @@ class padepoisson:
             self.destroy()
         except Exception:
             pass
+
+
+class Pade6stagg:
+    """type(Pade6stagg) of src/incompressible/PadeDerOps.F90 with scheme cd06:
+    ddz_C2E(input, output, bot, top) etc. on (n1, n2, nz) cell / (n1, n2, nz+1)
+    edge fields, bot / top in {-1 odd, 0 one-sided, 1 even} (ignored when
+    periodic)."""
+
+    KINDS = {"ddz_E2C": (10, 1, 0), "ddz_C2E": (11, 0, 1), "d2dz2_C2C": (12, 0, 0), "d2dz2_E2E": (13, 1, 1),
+             "interpz_E2C": (14, 1, 0), "interpz_C2E": (15, 0, 1)}
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def init(self, nz, dz, isPeriodic):
+        self.nz, self.dz = nz, dz
+        return _check(lib().pdo_pade6stagg_init(ctypes.byref(self._h), nz, dz, int(bool(isPeriodic))))
+
+    def _ap(self, name, inp, out, bot, top, n1, n2, is_complex, prec, stream):
+        kind, ein, eout = self.KINDS[name]
+        k = 2 if is_complex else 1
+        if _reals(inp) < k * n1 * n2 * (self.nz + ein) or _reals(out) < k * n1 * n2 * (self.nz + eout):
+            raise ValueError("buffer smaller than the declared shape")
+        return _check(lib().pdo_pade6stagg_apply(self._h, kind, bot, top, _prec(inp, prec), _ptr(inp), _ptr(out), n1,
+                                                 n2, int(is_complex), _stream(stream)))
+
+    def ddz_E2C(self, inp, out, bot, top, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        return self._ap("ddz_E2C", inp, out, bot, top, n1, n2, is_complex, prec, stream)
+
+    def ddz_C2E(self, inp, out, bot, top, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        return self._ap("ddz_C2E", inp, out, bot, top, n1, n2, is_complex, prec, stream)
+
+    def d2dz2_C2C(self, inp, out, bot, top, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        return self._ap("d2dz2_C2C", inp, out, bot, top, n1, n2, is_complex, prec, stream)
+
+    def d2dz2_E2E(self, inp, out, bot, top, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        return self._ap("d2dz2_E2E", inp, out, bot, top, n1, n2, is_complex, prec, stream)
+
+    def interpz_E2C(self, inp, out, bot, top, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        return self._ap("interpz_E2C", inp, out, bot, top, n1, n2, is_complex, prec, stream)
+
+    def interpz_C2E(self, inp, out, bot, top, n1, n2, is_complex=False, prec="f32_refine", stream=None):
+        return self._ap("interpz_C2E", inp, out, bot, top, n1, n2, is_complex, prec, stream)
+
+    @staticmethod
+    def getModifiedWavenumbers(k, dz, scheme="cd06"):
+        import numpy as np
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        kp = np.zeros_like(k)
+        dp = ctypes.POINTER(ctypes.c_double)
+        _check(lib().pdo_pade6stagg_modk({"fd02": 0, "cd06": 1, "fourierColl": 2}[scheme], k.size, dz,
+                                         k.ctypes.data_as(dp), kp.ctypes.data_as(dp)))
+        return kp
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_pade6stagg_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class duidxj:
+    """igrid%compute_duidxj (src/incompressible/igrid.F90:2553-2683) over a cell
+    and an edge spectral object and a Pade6stagg object."""
+
+    NAMES = ("uhat", "vhat", "whatC", "what", "uEhat", "vEhat", "duidxjC", "duidxjE", "duidxjChat", "duidxjEhat",
+             "d2wdz2hatE", "d2udz2hatC", "d2vdz2hatC")
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def init(self, spC, spE, dzop, bc=(0,) * 12, inviscid=False):
+        self.spC, self.spE, self.dzop = spC, spE, dzop
+        arr = (ctypes.c_int * 12)(*[int(b) for b in bc])
+        return _check(lib().pdo_duidxj_init(ctypes.byref(self._h), spC._h, spE._h, dzop._h, arr, int(inviscid)))
+
+    def compute(self, prec="f32_refine", stream=None, **fields):
+        """fields: the tensors named in NAMES (the d2 outputs may be omitted when inviscid)."""
+        F = _lib.DuidxjFields()
+        first = None
+        for name in self.NAMES:
+            t = fields.get(name)
+            if t is not None:
+                setattr(F, name, _ptr(t).value)
+                first = first if first is not None else t
+        return _check(lib().pdo_duidxj_compute(self._h, _prec(first, prec), ctypes.cast(ctypes.pointer(F),
+                                                                                       ctypes.c_void_p),
+                                               _stream(stream)))
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_duidxj_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -67,6 +67,13 @@ SIGNATURES = {
                                    _c_int, _c_int, _c_int, _c_int], _c_int),
     "pdo_poisson_periodic_solve": ([_vp, _c_int, _vp, _vp, _vp], _c_int),
     "pdo_poisson_destroy": ([_vp], _c_int),
+    "pdo_pade6stagg_init": ([_pvp, _c_int, _c_double, _c_int], _c_int),
+    "pdo_pade6stagg_apply": ([_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _vp], _c_int),
+    "pdo_pade6stagg_modk": ([_c_int, _c_int, _c_double, _pdbl, _pdbl], _c_int),
+    "pdo_pade6stagg_destroy": ([_vp], _c_int),
+    "pdo_duidxj_init": ([_pvp, _vp, _vp, _vp, _pint, _c_int], _c_int),
+    "pdo_duidxj_compute": ([_vp, _c_int, _vp, _vp], _c_int),
+    "pdo_duidxj_destroy": ([_vp], _c_int),
     "pdo_spectral_init": ([_pvp, _c_int, _c_int, _c_int, _c_double, _c_double, _c_double, _c_int, _c_double,
                            _c_int], _c_int),
     "pdo_spectral_fft": ([_vp, _c_int, _vp, _vp, _vp], _c_int),
@@ -86,6 +93,13 @@ SIGNATURES = {
     "pdo_rhs_destroy": ([_vp], _c_int),
     "pdo_rhs_max_divergence": ([_vp, _vp, _vp, _vp, _pdbl, _vp], _c_int),
 }
+
+
+class DuidxjFields(ctypes.Structure):
+    """pdo_duidxj_fields of include/padeops_hip.h"""
+    _fields_ = [(name, ctypes.c_void_p) for name in (
+        "uhat", "vhat", "whatC", "what", "uEhat", "vEhat", "duidxjC", "duidxjE", "duidxjChat", "duidxjEhat",
+        "d2wdz2hatE", "d2udz2hatC", "d2vdz2hatC")]
 
 
 class LibraryMissing(ImportError):

@@ -144,3 +144,41 @@ def test_projection_of_gradient_and_solenoidal_fields():
     assert np.abs(pu - su).max() < 1e-12 * np.abs(su).max()
     assert np.abs(pv - sv).max() < 1e-12 * np.abs(sv).max()
     assert np.abs(pw).max() < 1e-12 * np.abs(su).max()
+
+
+def _duidxj_case(nz, periodic):
+    from oracle.spectral import duidxj
+    nx, ny, Lz = 16, 16, 1.0
+    dx, dy, dz = L / nx, L / ny, Lz / nz
+    a = (2 * np.pi if periodic else np.pi) / Lz
+    x, y = _grid(nx), _grid(ny)
+    zc, ze = (np.arange(nz) + 0.5) * dz, np.arange(nz + 1) * dz
+    X, Y, ZC = np.meshgrid(x, y, zc, indexing="ij")
+    XE, YE, ZE = np.meshgrid(x, y, ze, indexing="ij")
+    u = lambda X, Y, Z: np.cos(X) * np.sin(2 * Y) * np.cos(a * Z)          # noqa: E731  even at the walls
+    w = lambda X, Y, Z: np.sin(2 * X + 1.0) * np.cos(Y) * np.sin(a * Z)    # noqa: E731  odd at the walls
+    spC, spE = Spectral(nx, ny, nz, dx, dy, dz), Spectral(nx, ny, nz + 1, dx, dy, dz)
+    bc = (1, 1, 1, 1, -1, -1, -1, -1, -1, -1, 1, 1)
+    o = duidxj(spC, spE, dz, periodic, bc, False, spC.fft(u(X, Y, ZC)), spC.fft(u(X, Y, ZC)), spC.fft(w(X, Y, ZC)),
+               spE.fft(w(XE, YE, ZE)), spE.fft(u(XE, YE, ZE)), spE.fft(u(XE, YE, ZE)))
+    errs = [
+        np.abs(o["duidxjE"][..., 2] + a * np.cos(XE) * np.sin(2 * YE) * np.sin(a * ZE)).max(),      # dudz on edges
+        np.abs(o["duidxjC"][..., 2] + a * np.cos(X) * np.sin(2 * Y) * np.sin(a * ZC)).max(),        # dudz on cells
+        np.abs(o["duidxjC"][..., 8] - a * np.sin(2 * X + 1) * np.cos(Y) * np.cos(a * ZC)).max(),   # dwdz
+        np.abs(spC.ifft(o["d2udz2hatC"]) + a * a * np.cos(X) * np.sin(2 * Y) * np.cos(a * ZC)).max(),
+    ]
+    exact_xy = max(np.abs(o["duidxjC"][..., 0] + np.sin(X) * np.sin(2 * Y) * np.cos(a * ZC)).max(),
+                   np.abs(o["duidxjE"][..., 7] + np.sin(2 * XE + 1) * np.sin(YE) * np.sin(a * ZE)).max())
+    return np.array(errs), exact_xy
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_duidxj_manufactured_solution(periodic):
+    """x/y derivatives are spectral (exact); the z derivatives, interpolations
+    and d2/dz2 converge at sixth order with the boundary kinds of the fields
+    (even u, odd w at the walls)."""
+    e16, x16 = _duidxj_case(16, periodic)
+    e32, x32 = _duidxj_case(32, periodic)
+    assert max(x16, x32) < 1e-13
+    ratio = e16 / e32
+    assert np.all(ratio > 50) and np.all(ratio < 80), ratio

@@ -168,3 +168,18 @@ def test_init_codes():
     assert o.init(16, 0.1) == 0
     with pytest.raises(P.PdoError):
         o.dense(16)
+
+
+def test_pade6stagg_modified_wavenumbers():
+    """getmodCD06stagg equals the oracle's modified wavenumber and the closed
+    form of the staggered symbol; fd02 is 2 sin(w/2)/dz; Fourier is k."""
+    from oracle.spectral import getwavenums
+    n, dz = 32, 2 * np.pi / 32
+    k = getwavenums(n, dz)
+    kp = P.Pade6stagg.getModifiedWavenumbers(k, dz, "cd06")
+    assert np.abs(kp - O.modk(2, n, dz)).max() < 1e-13
+    w = k * dz
+    sym = (2 * (63 / 62) * np.sin(w / 2) + (2 / 3) * (17 / 62) * np.sin(1.5 * w)) / (1 + 2 * (9 / 62) * np.cos(w)) / dz
+    assert np.abs(kp - sym).max() < 1e-13
+    assert np.abs(P.Pade6stagg.getModifiedWavenumbers(k, dz, "fd02") - 2 * np.sin(w / 2) / dz).max() < 1e-13
+    assert np.array_equal(P.Pade6stagg.getModifiedWavenumbers(k, dz, "fourierColl"), k)
