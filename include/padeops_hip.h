@@ -66,6 +66,7 @@ typedef struct pdo_spectral_s *pdo_spectral;
 typedef struct pdo_padepoisson_s *pdo_padepoisson;
 typedef struct pdo_pade6stagg_s *pdo_pade6stagg;
 typedef struct pdo_duidxj_s *pdo_duidxj;
+typedef struct pdo_dynproc_s *pdo_dynproc;
 
 /* ---- library ------------------------------------------------------------ */
 const char *pdo_last_error(void);
@@ -268,6 +269,29 @@ int pdo_duidxj_init(pdo_duidxj *h, pdo_spectral spC, pdo_spectral spE, pdo_pade6
                     int inviscid);
 int pdo_duidxj_compute(pdo_duidxj h, int prec, const pdo_duidxj_fields *fields, void *stream);
 int pdo_duidxj_destroy(pdo_duidxj h);
+
+/* ---- SGS eddy-viscosity models (src/incompressible/sgs_models) -------------
+ * pdo_sgs_kernel: get_Sij_from_duidxj (eddyViscosity.F90:1-21) fused with the
+ * model kernel, model 0 Smagorinsky |S| = sqrt(2 S_ij S_ij)
+ * (smagorinsky.F90:45-68) or 2 AMD with constants camd_x/y/z
+ * (AMD.F90:23-73); duidxj (npts, 9), Sij (npts, 6), nu_sgs (npts); Sij or
+ * nu_sgs may be NULL.
+ * pdo_sgs_scale: multiply_by_model_constant (eddyViscosity.F90:74-103):
+ * nu *= cplane[k] on plane k of nxy points (device array, e.g. the dynamic
+ * procedure's lambda), or cglobal when cplane is NULL.
+ * pdo_dynproc_*: DoStandardDynamicProcedure (standardDynamicProcedure.F90:
+ * 74-250) with the spectral test filter of InitTestFilter (spectral.F90:
+ * 1333-1367, filtfact = deltaRat) and no vertical test filter: from u, v, w
+ * (nx, ny, nz), their spectra, Sij and the unscaled Smagorinsky kernel Dsgs,
+ * the planar-averaged coefficient lambda_1d (device, nz FP64 values). */
+int pdo_sgs_kernel(int prec, int model, const void *duidxj, void *Sij, void *nu_sgs, long npts, double camd_x,
+                   double camd_y, double camd_z, void *stream);
+int pdo_sgs_scale(int prec, void *nu_sgs, long nxy, int nz, const double *cplane, double cglobal, void *stream);
+int pdo_dynproc_init(pdo_dynproc *h, pdo_spectral spC, double deltaRat);
+int pdo_dynproc_compute(pdo_dynproc h, int prec, const void *u, const void *v, const void *w, const void *uhat,
+                        const void *vhat, const void *what, const void *Sij, const void *Dsgs, double *lambda_1d,
+                        void *stream);
+int pdo_dynproc_destroy(pdo_dynproc h);
 
 /* ---- full RHS evaluation (the benchmarked composite) ----------------------
  * On a triply periodic n^3 grid of side L, for velocity (u, v, w):

@@ -554,3 +554,47 @@ class duidxj:
             self.destroy()
         except Exception:
             pass
+
+
+def sgs_kernel(duidxj, Sij, nu_sgs, npts, model="smagorinsky", camd=(0.0, 0.0, 0.0), prec="f32_refine",
+               stream=None):
+    """get_Sij_from_duidxj + get_smagorinsky_kernel / get_amd_kernel (sgs_models)."""
+    m = {"smagorinsky": 0, "amd": 2}[model]
+    for t, k in ((duidxj, 9), (Sij, 6), (nu_sgs, 1)):
+        if t is not None and t.numel() < k * npts:
+            raise ValueError("buffer smaller than the declared shape")
+    return _check(lib().pdo_sgs_kernel(_prec(duidxj, prec), m, _ptr(duidxj), _ptr(Sij), _ptr(nu_sgs), npts,
+                                       camd[0], camd[1], camd[2], _stream(stream)))
+
+
+def sgs_scale(nu_sgs, nxy, nz, cplane=None, cglobal=1.0, prec="f32_refine", stream=None):
+    """multiply_by_model_constant: cplane is a float64 CUDA tensor of nz values or None."""
+    return _check(lib().pdo_sgs_scale(_prec(nu_sgs, prec), _ptr(nu_sgs), nxy, nz, _ptr(cplane), cglobal,
+                                      _stream(stream)))
+
+
+class DynamicProcedure:
+    """DoStandardDynamicProcedure of sgs_igrid (standardDynamicProcedure.F90)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def init(self, spC, deltaRat=2.0):
+        self.sp = spC
+        return _check(lib().pdo_dynproc_init(ctypes.byref(self._h), spC._h, deltaRat))
+
+    def compute(self, u, v, w, uhat, vhat, what, Sij, Dsgs, lambda_1d, prec="f32_refine", stream=None):
+        return _check(lib().pdo_dynproc_compute(self._h, _prec(u, prec), _ptr(u), _ptr(v), _ptr(w), _ptr(uhat),
+                                                _ptr(vhat), _ptr(what), _ptr(Sij), _ptr(Dsgs), _ptr(lambda_1d),
+                                                _stream(stream)))
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_dynproc_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

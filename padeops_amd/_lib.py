@@ -74,6 +74,12 @@ SIGNATURES = {
     "pdo_duidxj_init": ([_pvp, _vp, _vp, _vp, _pint, _c_int], _c_int),
     "pdo_duidxj_compute": ([_vp, _c_int, _vp, _vp], _c_int),
     "pdo_duidxj_destroy": ([_vp], _c_int),
+    "pdo_sgs_kernel": ([_c_int, _c_int, _vp, _vp, _vp, ctypes.c_long, _c_double, _c_double, _c_double, _vp],
+                       _c_int),
+    "pdo_sgs_scale": ([_c_int, _vp, ctypes.c_long, _c_int, _vp, _c_double, _vp], _c_int),
+    "pdo_dynproc_init": ([_pvp, _vp, _c_double], _c_int),
+    "pdo_dynproc_compute": ([_vp, _c_int] + [_vp] * 9 + [_vp], _c_int),
+    "pdo_dynproc_destroy": ([_vp], _c_int),
     "pdo_spectral_init": ([_pvp, _c_int, _c_int, _c_int, _c_double, _c_double, _c_double, _c_int, _c_double,
                            _c_int], _c_int),
     "pdo_spectral_fft": ([_vp, _c_int, _vp, _vp, _vp], _c_int),

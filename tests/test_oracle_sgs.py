@@ -1,0 +1,78 @@
+"""Pins of the SGS oracle (oracle/sgs.py): closed forms of the Smagorinsky and
+AMD kernels, an independent tensor formulation of both, and invariances of
+the dynamic procedure (an identity test filter gives no resolved stress; the
+coefficient does not depend on the velocity scale)."""
+import numpy as np
+
+import oracle.sgs as G
+from oracle.spectral import Spectral
+
+
+def _grad(shape, seed):
+    return np.random.default_rng(seed).standard_normal(shape + (9,))
+
+
+def test_sij_is_symmetric_part():
+    g = _grad((5, 4, 3), 0)
+    S = G.sij(g)
+    T = g.reshape(g.shape[:-1] + (3, 3))          # T[..., i, j] = d u_i / d x_j
+    Sym = 0.5 * (T + np.swapaxes(T, -1, -2))
+    idx = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    for c, (i, j) in enumerate(idx):
+        assert np.allclose(S[..., c], Sym[..., i, j], rtol=0, atol=1e-15)
+
+
+def test_smagorinsky_closed_forms():
+    g = np.zeros((1, 9))
+    g[0, 1] = 3.0                                  # pure shear du/dy = 3: |S| = 3
+    assert abs(G.smagorinsky(G.sij(g))[0] - 3.0) < 1e-15
+    g = _grad((6, 5, 4), 1)
+    T = g.reshape(g.shape[:-1] + (3, 3))
+    Sym = 0.5 * (T + np.swapaxes(T, -1, -2))
+    assert np.allclose(G.smagorinsky(G.sij(g)), np.sqrt(2 * np.einsum("...ij,...ij", Sym, Sym)), rtol=1e-14)
+
+
+def test_amd_closed_forms():
+    cx, cy, cz = 0.3, 0.2, 0.5
+    g = np.zeros((1, 9))
+    g[0, 1] = 2.0                                  # pure shear: AMD vanishes
+    assert G.amd(g, G.sij(g), cx, cy, cz)[0] == 0.0
+    a = 0.7                                        # axisymmetric strain diag(-a, -a, 2a)
+    g = np.zeros((1, 9))
+    g[0, 0], g[0, 4], g[0, 8] = -a, -a, 2 * a
+    num = -a ** 3 * (cx ** 2 + cy ** 2) + 8 * a ** 3 * cz ** 2
+    assert abs(G.amd(g, G.sij(g), cx, cy, cz)[0] - max(-num / (6 * a * a), 0.0)) < 1e-15
+    g = _grad((4, 4, 4), 2)
+    T = g.reshape(g.shape[:-1] + (3, 3))
+    Sym = 0.5 * (T + np.swapaxes(T, -1, -2))
+    C2 = np.array([cx, cy, cz]) ** 2
+    ref = np.maximum(-np.einsum("...ik,...jk,k,...ij->...", T, T, C2, Sym) / (np.einsum("...ij,...ij", T, T) + 1e-32),
+                     0.0)
+    assert np.allclose(G.amd(g, G.sij(g), cx, cy, cz), ref, rtol=1e-13, atol=1e-15)
+
+
+def _dyn_fields(scale=1.0, n=16, nz=4, seed=3):
+    rng = np.random.default_rng(seed)
+    dx = 2 * np.pi / n
+    sp = Spectral(n, n, nz, dx, dx, 0.1)
+    u, v, w = (scale * rng.standard_normal((n, n, nz)) for _ in range(3))
+    g = scale * rng.standard_normal((n, n, nz, 9))
+    S = G.sij(g)
+    return sp, dx, u, v, w, S, G.smagorinsky(S)
+
+
+def test_dynamic_identity_filter_gives_zero():
+    sp, dx, u, v, w, S, D = _dyn_fields()
+    lam = G.dynamic_procedure(sp, dx, dx, 1e-3, u, v, w, sp.fft(u), sp.fft(v), sp.fft(w), S, D)
+    lam2 = G.dynamic_procedure(sp, dx, dx, 2.0, u, v, w, sp.fft(u), sp.fft(v), sp.fft(w), S, D)
+    # the resolved stress L_ij = F(u_i u_j) - F(u_i) F(u_j) cancels to round-off
+    assert np.all(lam <= 1e-12 * lam2.max()) and lam2.max() > 0
+
+
+def test_dynamic_scale_invariance():
+    sp, dx, u, v, w, S, D = _dyn_fields(1.0)
+    lam1 = G.dynamic_procedure(sp, dx, dx, 2.0, u, v, w, sp.fft(u), sp.fft(v), sp.fft(w), S, D)
+    sp, dx, u, v, w, S, D = _dyn_fields(3.0)
+    lam3 = G.dynamic_procedure(sp, dx, dx, 2.0, u, v, w, sp.fft(u), sp.fft(v), sp.fft(w), S, D)
+    assert np.any(lam1 > 0)
+    assert np.allclose(lam1, lam3, rtol=1e-10, atol=0)
