@@ -67,6 +67,7 @@ typedef struct pdo_padepoisson_s *pdo_padepoisson;
 typedef struct pdo_pade6stagg_s *pdo_pade6stagg;
 typedef struct pdo_duidxj_s *pdo_duidxj;
 typedef struct pdo_dynproc_s *pdo_dynproc;
+typedef struct pdo_mlp_s *pdo_mlp;
 
 /* ---- library ------------------------------------------------------------ */
 const char *pdo_last_error(void);
@@ -292,6 +293,29 @@ int pdo_dynproc_compute(pdo_dynproc h, int prec, const void *u, const void *v, c
                         const void *vhat, const void *what, const void *Sij, const void *Dsgs, double *lambda_1d,
                         void *stream);
 int pdo_dynproc_destroy(pdo_dynproc h);
+
+/* ---- learned SGS closure -----------------------------------------------------
+ * No counterpart in the reference: the pointwise MLP closure of this build
+ * (10 inputs -> 64 -> 64 -> 64 -> 6, GELU), trained and run on the device
+ * with BF16 weights / activations and FP32 accumulation on the matrix cores,
+ * FP32 master parameters and Adam.  Parameters are one FP32 vector of
+ * pdo_mlp_nparam() values: W1 (64 x 16, columns 10..15 zero), b1 (64),
+ * W2 (64 x 64), b2, W3 (64 x 64), b3, W4 (16 x 64, rows 6..15 zero), b4 (16,
+ * entries 6..15 zero), each W row-major [out][in].  X (B, 10) and the
+ * targets T / outputs Y (B, 6) are row-major FP32 device arrays.  The loss is
+ * the mean over B x 6 of the squared error; gradients are summed over the
+ * batch in a fixed order (deterministic).  set_params resets Adam. */
+int pdo_mlp_nparam(void);
+int pdo_mlp_create(pdo_mlp *h, long max_batch);
+int pdo_mlp_set_params(pdo_mlp h, const float *host_params);
+int pdo_mlp_get_params(pdo_mlp h, float *host_params);
+int pdo_mlp_get_grads(pdo_mlp h, float *host_grads);
+int pdo_mlp_forward(pdo_mlp h, const float *X, float *Y, long B, void *stream);
+int pdo_mlp_gradients(pdo_mlp h, const float *X, const float *T, long B, float *Y, void *stream);
+int pdo_mlp_adam_step(pdo_mlp h, float lr, float b1, float b2, float eps, void *stream);
+int pdo_mlp_train_step(pdo_mlp h, const float *X, const float *T, long B, float lr, float b1, float b2, float eps,
+                       float *Y, void *stream);
+int pdo_mlp_destroy(pdo_mlp h);
 
 /* ---- full RHS evaluation (the benchmarked composite) ----------------------
  * On a triply periodic n^3 grid of side L, for velocity (u, v, w):

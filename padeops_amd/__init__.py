@@ -598,3 +598,76 @@ class DynamicProcedure:
             self.destroy()
         except Exception:
             pass
+
+
+class SGSNet:
+    """The learned SGS closure network (pdo_mlp_* in the C header): BF16 MFMA
+    forward / backward kernels with FP32 accumulation and Adam."""
+
+    # offsets of the padded FP32 parameter vector
+    LAYOUT = (("W1", 0, (64, 16)), ("b1", 1024, (64,)), ("W2", 1088, (64, 64)), ("b2", 5184, (64,)),
+              ("W3", 5248, (64, 64)), ("b3", 9344, (64,)), ("W4", 9408, (16, 64)), ("b4", 10432, (16,)))
+
+    def __init__(self, max_batch):
+        self._h = ctypes.c_void_p()
+        self.nparam = lib().pdo_mlp_nparam()
+        _check(lib().pdo_mlp_create(ctypes.byref(self._h), max_batch))
+
+    @classmethod
+    def pack(cls, params):
+        """[(W (out, in), b (out,))] x 4 of the 10-64-64-64-6 network -> padded FP32 vector."""
+        import numpy as np
+        v = np.zeros(10448, np.float32)
+        for (name, off, shape), arr in zip(cls.LAYOUT, [a for wb in params for a in wb]):
+            full = np.zeros(shape, np.float32)
+            a = np.asarray(arr, np.float32)
+            full[tuple(slice(0, s) for s in a.shape)] = a
+            v[off:off + full.size] = full.ravel()
+        return v
+
+    @classmethod
+    def unpack(cls, v, sizes=(10, 64, 64, 64, 6)):
+        out = []
+        for i in range(4):
+            (_, ow, sw), (_, ob, sb) = cls.LAYOUT[2 * i], cls.LAYOUT[2 * i + 1]
+            W = v[ow:ow + sw[0] * sw[1]].reshape(sw)[: sizes[i + 1], : sizes[i]]
+            out.append((W.copy(), v[ob:ob + sb[0]][: sizes[i + 1]].copy()))
+        return out
+
+    def set_params(self, vec):
+        import numpy as np
+        vec = np.ascontiguousarray(vec, np.float32)
+        _check(lib().pdo_mlp_set_params(self._h, vec.ctypes.data_as(ctypes.c_void_p)))
+
+    def get_params(self):
+        import numpy as np
+        v = np.zeros(self.nparam, np.float32)
+        _check(lib().pdo_mlp_get_params(self._h, v.ctypes.data_as(ctypes.c_void_p)))
+        return v
+
+    def get_grads(self):
+        import numpy as np
+        v = np.zeros(self.nparam, np.float32)
+        _check(lib().pdo_mlp_get_grads(self._h, v.ctypes.data_as(ctypes.c_void_p)))
+        return v
+
+    def forward(self, X, Y, stream=None):
+        return _check(lib().pdo_mlp_forward(self._h, _ptr(X), _ptr(Y), X.shape[0], _stream(stream)))
+
+    def gradients(self, X, T, Y=None, stream=None):
+        return _check(lib().pdo_mlp_gradients(self._h, _ptr(X), _ptr(T), X.shape[0], _ptr(Y), _stream(stream)))
+
+    def train_step(self, X, T, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, Y=None, stream=None):
+        return _check(lib().pdo_mlp_train_step(self._h, _ptr(X), _ptr(T), X.shape[0], lr, b1, b2, eps, _ptr(Y),
+                                               _stream(stream)))
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

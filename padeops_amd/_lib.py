@@ -80,6 +80,17 @@ SIGNATURES = {
     "pdo_dynproc_init": ([_pvp, _vp, _c_double], _c_int),
     "pdo_dynproc_compute": ([_vp, _c_int] + [_vp] * 9 + [_vp], _c_int),
     "pdo_dynproc_destroy": ([_vp], _c_int),
+    "pdo_mlp_nparam": ([], _c_int),
+    "pdo_mlp_create": ([_pvp, ctypes.c_long], _c_int),
+    "pdo_mlp_set_params": ([_vp, _vp], _c_int),
+    "pdo_mlp_get_params": ([_vp, _vp], _c_int),
+    "pdo_mlp_get_grads": ([_vp, _vp], _c_int),
+    "pdo_mlp_forward": ([_vp, _vp, _vp, ctypes.c_long, _vp], _c_int),
+    "pdo_mlp_gradients": ([_vp, _vp, _vp, ctypes.c_long, _vp, _vp], _c_int),
+    "pdo_mlp_adam_step": ([_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp], _c_int),
+    "pdo_mlp_train_step": ([_vp, _vp, _vp, ctypes.c_long, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                            ctypes.c_float, _vp, _vp], _c_int),
+    "pdo_mlp_destroy": ([_vp], _c_int),
     "pdo_spectral_init": ([_pvp, _c_int, _c_int, _c_int, _c_double, _c_double, _c_double, _c_int, _c_double,
                            _c_int], _c_int),
     "pdo_spectral_fft": ([_vp, _c_int, _vp, _vp, _vp], _c_int),

@@ -1,0 +1,278 @@
+// Learned SGS closure: a pointwise MLP 10 -> 64 -> 64 -> 64 -> 6 (GELU on the
+// hidden layers), trained and evaluated with hand-written gfx950 kernels:
+// BF16 weights and activations, FP32 accumulation on the matrix cores
+// (v_mfma_f32_16x16x16_bf16), FP32 master parameters updated by Adam.
+//
+// A workgroup (4 waves) owns a tile of 64 samples and runs the whole network
+// on it with every weight matrix staged in LDS; wave w owns rows 16w..16w+15
+// of the activation tile.  The backward kernel forms per-workgroup weight
+// gradients (the batch is the contraction dimension of dW = dZ^T H) that a
+// second kernel sums in a fixed order, so training is deterministic.
+#include <hip/hip_runtime.h>
+
+#include "pdo_mlp.hpp"
+
+namespace pdo {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int TILE = 64;     // samples per workgroup
+constexpr int HS = 64 + 8;   // LDS row pitch of activation tiles (bf16)
+
+__device__ __forceinline__ unsigned short to_bf16(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+__device__ __forceinline__ float gelu_f(float z)
+{
+    const float t = tanhf(0.7978845608028654f * (z + 0.044715f * z * z * z));
+    return 0.5f * z * (1.0f + t);
+}
+__device__ __forceinline__ float gelu_grad_f(float z)
+{
+    const float t = tanhf(0.7978845608028654f * (z + 0.044715f * z * z * z));
+    const float du = 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * z * z);
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * du;
+}
+
+// One 16x16 output tile of this wave: D(m, n) += sum_k A(m, k) B(k, n),
+// K a multiple of 16.  Operand layout of the 16x16x16 BF16 MFMA: lane l
+// holds A(l % 16, 4 (l / 16) + i) and B(4 (l / 16) + i, l % 16), i < 4; the
+// result D(4 (l / 16) + i, l % 16) lands in element i of lane l.
+template <class AF, class BF>
+__device__ __forceinline__ f32x4 mma16(AF a, BF b, int K, f32x4 acc)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, g = lane >> 4;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf16x4 av, bv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            av[i] = (short)a(r, k0 + 4 * g + i);
+            bv[i] = (short)b(k0 + 4 * g + i, r);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+struct Layers {
+    int in[4] = {MLP_IN_PAD, MLP_HID, MLP_HID, MLP_HID};
+    int out[4] = {MLP_HID, MLP_HID, MLP_HID, MLP_OUT_PAD};
+    int w[4] = {MLP_OFF_W1, MLP_OFF_W2, MLP_OFF_W3, MLP_OFF_W4};
+    int b[4] = {MLP_OFF_B1, MLP_OFF_B2, MLP_OFF_B3, MLP_OFF_B4};
+};
+
+// stage every weight matrix (BF16) of the network in LDS
+__device__ void stage_weights(unsigned short *sW, const unsigned short *Wb)
+{
+    for (int i = threadIdx.x; i < MLP_NPARAM; i += blockDim.x) sW[i] = Wb[i];
+}
+
+// ---------------------------------------------------------------------------
+// forward: X (B, 10) FP32 -> Y (B, 6) FP32; when cache != null the BF16 layer
+// inputs H0 (B, 16), H1..H3 (B, 64) and pre-activations Z1..Z3 (B, 64) are
+// kept for the backward pass
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_mlp_forward(const float *__restrict__ X, float *__restrict__ Y,
+                                                    const unsigned short *__restrict__ Wb,
+                                                    const float *__restrict__ P, long B, MlpCache cache)
+{
+    __shared__ unsigned short sW[MLP_NPARAM];
+    __shared__ unsigned short sH[2][TILE * HS];
+    const Layers L;
+    const long row0 = (long)blockIdx.x * TILE;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    stage_weights(sW, Wb);
+    for (int i = threadIdx.x; i < TILE * MLP_IN_PAD; i += blockDim.x) {
+        const int m = i / MLP_IN_PAD, k = i % MLP_IN_PAD;
+        const long s = row0 + m;
+        const float x = (s < B && k < MLP_IN) ? X[s * MLP_IN + k] : 0.0f;
+        const unsigned short hb = to_bf16(x);
+        sH[0][m * HS + k] = hb;
+        if (cache.H[0] && s < B) cache.H[0][s * MLP_IN_PAD + k] = hb;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int l = 0; l < 4; ++l) {
+        const unsigned short *Wl = sW + L.w[l];
+        const int K = L.in[l], N = L.out[l];
+        const unsigned short *Hc = sH[cur];
+        for (int nt = 0; nt < N / 16; ++nt) {
+            f32x4 acc = {0, 0, 0, 0};
+            acc = mma16([&](int m, int k) { return Hc[(16 * w + m) * HS + k]; },
+                        [&](int k, int n) { return Wl[(16 * nt + n) * K + k]; }, K, acc);
+            const int col = 16 * nt + r;
+            const float bias = P[L.b[l] + col];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = 16 * w + 4 * g + i;
+                const long s = row0 + m;
+                const float z = acc[i] + bias;
+                if (l < 3) {
+                    const unsigned short zb = to_bf16(z);
+                    const unsigned short hb = to_bf16(gelu_f(from_bf16(zb)));
+                    sH[cur ^ 1][m * HS + col] = hb;
+                    if (cache.Z[l] && s < B) {
+                        cache.Z[l][s * MLP_HID + col] = zb;
+                        cache.H[l + 1][s * MLP_HID + col] = hb;
+                    }
+                } else if (col < MLP_OUT && s < B) {
+                    Y[s * MLP_OUT + col] = z;
+                }
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// backward: per-workgroup gradient partials of the mean squared error
+// loss = mean over B x 6 of (Y - T)^2; partial[blockIdx.x][MLP_NPARAM]
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_mlp_backward(const float *__restrict__ Y, const float *__restrict__ T,
+                                                     const unsigned short *__restrict__ Wb, long B,
+                                                     MlpCache cache, float *__restrict__ partial)
+{
+    __shared__ unsigned short sW[MLP_NPARAM];
+    __shared__ unsigned short sD[TILE * HS];   // dZ (BF16), [sample][feature]
+    __shared__ float sDf[TILE * (64 + 1)];     // dZ (FP32) for the bias sums
+    __shared__ unsigned short sH[TILE * HS];   // layer input H, [sample][feature]
+    __shared__ unsigned short sZ[TILE * HS];   // pre-activation of the layer below
+    const Layers L;
+    const long row0 = (long)blockIdx.x * TILE;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    float *out = partial + (size_t)blockIdx.x * MLP_NPARAM;
+    stage_weights(sW, Wb);
+    const float scale = 2.0f / (float)(B * MLP_OUT);
+    for (int i = threadIdx.x; i < TILE * MLP_OUT_PAD; i += blockDim.x) {
+        const int m = i / MLP_OUT_PAD, n = i % MLP_OUT_PAD;
+        const long s = row0 + m;
+        const float d = (s < B && n < MLP_OUT) ? scale * (Y[s * MLP_OUT + n] - T[s * MLP_OUT + n]) : 0.0f;
+        sDf[m * 65 + n] = d;
+        sD[m * HS + n] = to_bf16(d);
+    }
+    for (int l = 3; l >= 0; --l) {
+        const int K = L.in[l], N = L.out[l];
+        const unsigned short *Wl = sW + L.w[l];
+        // layer input H_l (and Z_l of the layer below, for the GELU derivative)
+        const unsigned short *Hg = cache.H[l];
+        const unsigned short *Zg = l > 0 ? cache.Z[l - 1] : nullptr;
+        for (int i = threadIdx.x; i < TILE * K; i += blockDim.x) {
+            const int m = i / K, k = i % K;
+            const long s = row0 + m;
+            sH[m * HS + k] = s < B ? Hg[s * K + k] : (unsigned short)0;
+            if (Zg) sZ[m * HS + k] = s < B ? Zg[s * K + k] : (unsigned short)0;
+        }
+        __syncthreads();
+        // bias gradient: column sums of dZ over the tile
+        for (int n = threadIdx.x; n < N; n += blockDim.x) {
+            float acc = 0.0f;
+            for (int m = 0; m < TILE; ++m) acc += sDf[m * 65 + n];
+            out[L.b[l] + n] = acc;
+        }
+        // weight gradient dW (N x K) = dZ^T (N x TILE) * H (TILE x K)
+        for (int t = w; t < (N / 16) * (K / 16); t += 4) {
+            const int mt = t / (K / 16), nt = t % (K / 16);
+            f32x4 acc = {0, 0, 0, 0};
+            acc = mma16([&](int m, int k) { return sD[k * HS + 16 * mt + m]; },
+                        [&](int k, int n) { return sH[k * HS + 16 * nt + n]; }, TILE, acc);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) out[L.w[l] + (16 * mt + 4 * g + i) * K + 16 * nt + r] = acc[i];
+        }
+        if (l == 0) break;
+        // dH (TILE x K) = dZ (TILE x N) * W (N x K); dZ_below = dH * gelu'(Z)
+        f32x4 dh[4];
+        for (int nt = 0; nt < K / 16; ++nt) {
+            f32x4 acc = {0, 0, 0, 0};
+            dh[nt] = mma16([&](int m, int k) { return sD[(16 * w + m) * HS + k]; },
+                           [&](int k, int n) { return Wl[k * K + 16 * nt + n]; }, N, acc);
+        }
+        __syncthreads();  // every read of sD / sDf for this layer is done
+        for (int nt = 0; nt < K / 16; ++nt) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = 16 * w + 4 * g + i, col = 16 * nt + r;
+                const float d = dh[nt][i] * gelu_grad_f(from_bf16(sZ[m * HS + col]));
+                sDf[m * 65 + col] = d;
+                sD[m * HS + col] = to_bf16(d);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// grads[p] = sum over the nblk partials, in block order
+__global__ void __launch_bounds__(256) k_mlp_reduce(const float *__restrict__ partial, int nblk, float *grads)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= MLP_NPARAM) return;
+    float acc = 0.0f;
+    for (int b = 0; b < nblk; ++b) acc += partial[(size_t)b * MLP_NPARAM + p];
+    grads[p] = acc;
+}
+
+// Adam on the FP32 master parameters, then the BF16 copy used by the kernels
+__global__ void __launch_bounds__(256) k_mlp_adam(float *P, float *M, float *V, const float *grads,
+                                                 unsigned short *Wb, float lr, float b1, float b2, float eps,
+                                                 float c1, float c2)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= MLP_NPARAM) return;
+    const float gr = grads[p];
+    const float m = b1 * M[p] + (1.0f - b1) * gr;
+    const float v = b2 * V[p] + (1.0f - b2) * gr * gr;
+    M[p] = m;
+    V[p] = v;
+    const float np = P[p] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+    P[p] = np;
+    Wb[p] = to_bf16(np);
+}
+
+__global__ void k_mlp_to_bf16(const float *P, unsigned short *Wb)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < MLP_NPARAM) Wb[p] = to_bf16(P[p]);
+}
+
+static unsigned nblocks(long B) { return (unsigned)((B + TILE - 1) / TILE); }
+
+hipError_t launch_mlp_forward(const float *X, float *Y, const unsigned short *Wb, const float *P, long B,
+                              const MlpCache &cache, hipStream_t st)
+{
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_forward, dim3(nblocks(B)), dim3(256), 0, st, X, Y, Wb, P, B, cache);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_backward(const float *Y, const float *T, const unsigned short *Wb, long B,
+                               const MlpCache &cache, float *partial, float *grads, hipStream_t st)
+{
+    if (B <= 0) return hipSuccess;
+    const unsigned nb = nblocks(B);
+    hipLaunchKernelGGL(k_mlp_backward, dim3(nb), dim3(256), 0, st, Y, T, Wb, B, cache, partial);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, partial, (int)nb, grads);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(float *P, float *M, float *V, const float *grads, unsigned short *Wb, float lr, float b1,
+                           float b2, float eps, float c1, float c2, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, P, M, V, grads, Wb, lr, b1, b2,
+                       eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_to_bf16(const float *P, unsigned short *Wb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mlp_to_bf16, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, P, Wb);
+    return hipGetLastError();
+}
+
+}  // namespace pdo
