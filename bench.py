@@ -25,31 +25,8 @@ WORDS_PER_POINT = 65  # algorithmic words per RHS evaluation
 
 def turbulence(n, k0, seed, dtype, device):
     """Divergence-free random-phase field with E(k) ~ k^4 exp(-2 (k/k0)^2), rms 1."""
-    import torch
-    g = torch.Generator(device=device)
-    g.manual_seed(seed)
-    k1 = torch.fft.fftfreq(n, d=1.0 / n, device=device, dtype=torch.float64)
-    kr = torch.fft.rfftfreq(n, d=1.0 / n, device=device, dtype=torch.float64)
-    KX = k1.view(n, 1, 1)
-    KY = k1.view(1, n, 1)
-    KZ = kr.view(1, 1, -1)
-    K2 = KX * KX + KY * KY + KZ * KZ
-    Km = torch.sqrt(K2).clamp_min(1.0)
-    amp = torch.sqrt(Km ** 4 * torch.exp(-2.0 * (Km / k0) ** 2) / (4 * torch.pi * Km ** 2))
-    amp[0, 0, 0] = 0.0
-    uh = []
-    for _ in range(3):
-        re = torch.randn(K2.shape, generator=g, device=device, dtype=torch.float64)
-        im = torch.randn(K2.shape, generator=g, device=device, dtype=torch.float64)
-        uh.append(torch.complex(re, im) * amp)
-    kd = (KX * uh[0] + KY * uh[1] + KZ * uh[2]) / torch.where(K2 == 0, torch.ones_like(K2), K2)
-    uh = [uh[0] - KX * kd, uh[1] - KY * kd, uh[2] - KZ * kd]
-    # torch axes (a0, a1, a2) with a2 fastest are the Fortran (z, y, x): the
-    # x velocity is the component along a2
-    u = [torch.fft.irfftn(c, s=(n, n, n)) for c in uh]
-    del uh, kd
-    rms = torch.sqrt((u[0] ** 2 + u[1] ** 2 + u[2] ** 2).mean() / 3.0)
-    return [(c / rms).contiguous().view(-1).to(dtype) for c in (u[2], u[1], u[0])]
+    from padeops_amd.synthetic import turbulence as gen
+    return gen(n, k0, seed, dtype, device)
 
 
 def rel_l2(a, b):
@@ -67,6 +44,10 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-breakdown", action="store_true", help="skip the per-pass timing loop")
+    ap.add_argument("--profile-pass", default=None,
+                    help="only launch this pass type (cd06_x .. cf90_z, poisson_5pass) --launches times, then a "
+                         "1 GiB device copy as many times (the byte-counter calibration of rocprofv3 --pmc runs)")
+    ap.add_argument("--launches", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -80,6 +61,9 @@ def main():
 
     dev = torch.device("cuda:0")
     n = args.n
+    if args.profile_pass:
+        profile_pass(P, n, dev, args.profile_pass, args.launches)
+        return
     N = n ** 3
     L = 2 * np.pi
     stream = torch.cuda.current_stream().cuda_stream
@@ -191,13 +175,11 @@ def main():
     print(json.dumps(line))
 
 
-def pass_breakdown(P, n, f32, stream, steps):
-    """Average duration of each pass type of the FP32+refine evaluation, timed
-    with HIP events on the launch stream; returns (ms per pass, roofline of
-    the dominant kernel)."""
+def pass_calls(P, n, f32, stream):
+    """One launch of each pass type of the FP32+refine evaluation, and how
+    often each runs per evaluation."""
     import numpy as np
     import torch
-    N = n ** 3
     out = torch.empty_like(f32)
     L = 2 * np.pi
     d = P.cd06()
@@ -217,6 +199,16 @@ def pass_breakdown(P, n, f32, stream, steps):
     }
     # uses per evaluation of each pass type
     uses = {"cd06_x": 5, "cd06_y": 5, "cd06_z": 5, "cf90_x": 3, "cf90_y": 3, "cf90_z": 3, "poisson_5pass": 1}
+    return calls, uses, (d, c, ps, out)
+
+
+def pass_breakdown(P, n, f32, stream, steps):
+    """Average duration of each pass type of the FP32+refine evaluation, timed
+    with HIP events on the launch stream; returns (ms per pass, roofline of
+    the dominant kernel)."""
+    import torch
+    N = n ** 3
+    calls, uses, _keep = pass_calls(P, n, f32, stream)
     res = {}
     for name, fn in calls.items():
         for _ in range(2):
@@ -237,6 +229,24 @@ def pass_breakdown(P, n, f32, stream, steps):
             "frac": round(ach / PEAK_HBM, 4), "traffic": None,
             "bytes_per_launch": algo, "avg_launch_ms": round(res[dom], 5)}
     return {k: round(v, 5) for k, v in res.items()}, roof
+
+
+def profile_pass(P, n, dev, name, launches):
+    """Launches of one pass type and of a plain 1 GiB copy, nothing else timed:
+    the command rocprofv3 counter runs profile."""
+    import torch
+    stream = torch.cuda.current_stream().cuda_stream
+    f32 = turbulence(n, 8.0, 1234, torch.float32, dev)[0]
+    calls, _, _keep = pass_calls(P, n, f32, stream)
+    fn = calls[name]
+    for _ in range(launches):
+        fn()
+    a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(launches):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile_pass": name, "launches": launches, "n": n, "copy_bytes_each_way": 4 << 28}))
 
 
 def hbm_probe(steps):

@@ -229,6 +229,11 @@ int pdo_spectral_dealias_edge(pdo_spectral h, int prec, void *fhat, void *stream
  * g3 (0 or 1, the mask is their product); NULL skips an array */
 int pdo_spectral_wavenumbers(pdo_spectral h, double *k1, double *k2, double *k3, double *g1, double *g2,
                              double *g3);
+/* replace the dealias factors g1 (nx/2+1), g2 (ny), g3 (nz) by any separable
+ * real transfer function (host arrays; NULL keeps one): pdo_spectral_dealias
+ * then applies that filter, e.g. a Gaussian LES filter.  No reference
+ * counterpart (the reference's masks are fixed at init). */
+int pdo_spectral_set_factors(pdo_spectral h, const double *g1, const double *g2, const double *g3);
 int pdo_spectral_destroy(pdo_spectral h);
 
 /* ---- PadePoisson: periodic pressure projection of igrid ---------------------

@@ -389,6 +389,14 @@ class spectral:
             raise ValueError("buffer smaller than the declared shape")
         return _check(lib().pdo_spectral_dealias_edge(self._h, _prec(fhat, prec), _ptr(fhat), _stream(stream)))
 
+    def set_factors(self, g1=None, g2=None, g3=None):
+        """Replace the dealias factors by any separable transfer function."""
+        import numpy as np
+        dp = ctypes.POINTER(ctypes.c_double)
+        keep = [None if g is None else np.ascontiguousarray(g, np.float64) for g in (g1, g2, g3)]
+        args = [None if g is None else g.ctypes.data_as(dp) for g in keep]
+        return _check(lib().pdo_spectral_set_factors(self._h, *args))
+
     def wavenumbers(self):
         """(k1, k2, k3, g1, g2, g3) host arrays; the dealias mask is g1 g2 g3."""
         import numpy as np

@@ -99,6 +99,7 @@ SIGNATURES = {
     "pdo_spectral_dealias": ([_vp, _c_int, _vp, _vp], _c_int),
     "pdo_spectral_dealias_edge": ([_vp, _c_int, _vp, _vp], _c_int),
     "pdo_spectral_wavenumbers": ([_vp] + [_pdbl] * 6, _c_int),
+    "pdo_spectral_set_factors": ([_vp, _pdbl, _pdbl, _pdbl], _c_int),
     "pdo_spectral_destroy": ([_vp], _c_int),
     "pdo_padepoisson_init_periodic": ([_pvp, _vp, _c_double], _c_int),
     "pdo_padepoisson_projection": ([_vp, _c_int, _vp, _vp, _vp, _vp], _c_int),

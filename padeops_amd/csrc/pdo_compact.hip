@@ -78,15 +78,30 @@ __device__ __forceinline__ T stencil_row(const TW (&win)[W], int j, const T *c)
     return acc;
 }
 
+// right closure row against input columns [n_in - WBMAX, n_in): window index
+// MAXM + R - WBMAX + q + DN with DN = n_in - n_sys
+template <int FAM, int DN, typename T, typename TW, int W>
+__device__ __forceinline__ T right_closure(const TW (&win)[W], const double *sp)
+{
+    constexpr int base = MAXM + Fam<FAM>::R - WBMAX + DN;
+    static_assert(base >= 0 && base + WBMAX <= W, "closure window");
+    T acc = 0;
+#pragma unroll
+    for (int q = 0; q < WBMAX; ++q) acc = fma((T)sp[q], (T)win[base + q], acc);
+    return acc;
+}
+
 // RHS row j of segment `seg` with the closure rows of non-periodic systems.
 // Closure rows are stored WBMAX wide: left rows against input columns
-// [0, WBMAX), right rows against [n_in - WBMAX, n_in), so every window index is
-// a compile-time constant (n_in == n_sys for all non-periodic systems).
+// [0, WBMAX), right rows against [n_in - WBMAX, n_in).  Staggered systems
+// read one input point more (E2C: n_in = n_sys + 1) or less (C2E: n_in =
+// n_sys - 1) than they solve; the uniform branch on the difference keeps
+// every window index a compile-time constant.
 template <int FAM, typename T, typename TW, int W>
 __device__ __forceinline__ T rhs_row(const TW (&win)[W], int j, const T *c, const ApplyArgs &A, int ty, int m)
 {
     using F = Fam<FAM>;
-    static_assert(MAXM + F::R - WBMAX >= 0, "closure window");
+    static_assert(MAXM + F::R - WBMAX >= 1, "closure window");
     if (!A.periodic) {
         if ((ty == 0 || ty == 3) && j < A.nbl) {
             const double *sp = A.tab_d + A.o_sp + j * WBMAX;
@@ -97,10 +112,10 @@ __device__ __forceinline__ T rhs_row(const TW (&win)[W], int j, const T *c, cons
         }
         if ((ty == 2 || ty == 3) && j >= m - A.nbr) {
             const double *sp = A.tab_d + A.o_sp + (NBMAX + j - (m - A.nbr)) * WBMAX;
-            T acc = 0;
-#pragma unroll
-            for (int q = 0; q < WBMAX; ++q) acc = fma((T)sp[q], (T)win[MAXM + F::R - WBMAX + q], acc);
-            return acc;
+            const int dn = A.n_in - A.n_sys;
+            if (dn == 0) return right_closure<FAM, 0, T>(win, sp);
+            if (dn > 0) return right_closure<FAM, 1, T>(win, sp);
+            return right_closure<FAM, -1, T>(win, sp);
         }
     }
     return stencil_row<FAM, T>(win, j, c);
@@ -668,6 +683,12 @@ static hipError_t launch_strided_k(const ApplyArgs &A, hipStream_t st)
 // only used with the periodic cd06 first derivative
 template <int FAM> constexpr bool has_epilogue() { return FAM == 0; }
 
+// pencils per workgroup of the periodic strided kernel: 32 (two segments per
+// wave) or 64 (one); tuning constant
+#ifndef PDO_STRIDED_NPW_PER
+#define PDO_STRIDED_NPW_PER 32
+#endif
+
 template <int FAM, int MODE>
 static hipError_t launch_strided_t(const ApplyArgs &A, hipStream_t st)
 {
@@ -677,12 +698,12 @@ static hipError_t launch_strided_t(const ApplyArgs &A, hipStream_t st)
     }
     if (A.epi) {
         if constexpr (has_epilogue<FAM>()) {
-            if (A.P % 2 == 0) return launch_strided_k<FAM, MODE, 32, true, true>(A, st);
+            if (PDO_STRIDED_NPW_PER == 32 && A.P % 2 == 0) return launch_strided_k<FAM, MODE, 32, true, true>(A, st);
             return launch_strided_k<FAM, MODE, 64, true, true>(A, st);
         }
         return hipErrorInvalidValue;
     }
-    if (A.P % 2 == 0) return launch_strided_k<FAM, MODE, 32, true, false>(A, st);
+    if (PDO_STRIDED_NPW_PER == 32 && A.P % 2 == 0) return launch_strided_k<FAM, MODE, 32, true, false>(A, st);
     return launch_strided_k<FAM, MODE, 64, true, false>(A, st);
 }
 

@@ -372,12 +372,21 @@ static int ilog2(int v)
     return l;
 }
 
+// largest workgroups of the line transforms (threads); tuning constants
+#ifndef PDO_FFT_STRIDED_THREADS
+#define PDO_FFT_STRIDED_THREADS 1024
+#endif
+#ifndef PDO_FFT_X_THREADS
+#define PDO_FFT_X_THREADS 1024
+#endif
+
 template <typename TS, typename TC, int MODE, int L>
 static void launch_strided_fft(const TS *src, TS *dst, long na, long sb, long se, const TC *tw, const SpecMul &pm,
                                int nb, hipStream_t st)
 {
     const size_t ce = 2 * sizeof(TC);
-    int TX = 1024 / (L / 8);
+    int TX = PDO_FFT_STRIDED_THREADS / (L / 8);
+    if (TX < 1) TX = 1;
     while (TX > 1 && (size_t)TX * L * ce > 64 * 1024) TX >>= 1;
     int lg = 0;
     while ((1 << lg) < TX) lg++;
@@ -407,7 +416,8 @@ static void launch_x_fft(bool inverse, const TS *in, TS *out, long nlines, const
                          int nxp, int oddball, hipStream_t st)
 {
     const size_t ce = 2 * sizeof(TC);
-    int TXL = 1024 / (L / 8);
+    int TXL = PDO_FFT_X_THREADS / (L / 8);
+    if (TXL < 1) TXL = 1;
     while (TXL > 1 && (size_t)TXL * (L + 1) * ce > 64 * 1024) TXL >>= 1;
     dim3 grid((unsigned)((nlines + TXL - 1) / TXL));
     if (!inverse)

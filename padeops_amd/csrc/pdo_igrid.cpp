@@ -69,8 +69,10 @@ int pdo_duidxj_compute(pdo_duidxj h, int prec, const pdo_duidxj_fields *F, void 
     const int uB = bc[0], uT = bc[1], vB = bc[2], vT = bc[3], wB = bc[4], wT = bc[5];
     const int dUB = bc[6], dUT = bc[7], dVB = bc[8], dVT = bc[9], dWB = bc[10], dWT = bc[11];
     int st;
-#define PDO_TRY(x) \
-    if ((st = (x))) return st
+#define PDO_TRY(x)            \
+    do {                      \
+        if ((st = (x))) return st; \
+    } while (0)
     // x and y derivatives: i k then the 2D inverse transform, on cells and edges
     struct XY { const void *c, *e; int kC; };
     const XY xy[6] = {{F->uhat, F->uEhat, 1}, {F->uhat, F->uEhat, 2}, {F->vhat, F->vEhat, 4},
@@ -99,10 +101,11 @@ int pdo_duidxj_compute(pdo_duidxj h, int prec, const pdo_duidxj_fields *F, void 
         if (!h->inviscid) {
             // a one-sided boundary takes the second derivative as E2C(C2E(.));
             // the C2E result is the edge derivative just computed
-            if (q.t == 0 || q.b == 0)
+            if (q.t == 0 || q.b == 0) {
                 PDO_TRY(pdo_pade6stagg_apply(h->dz, 10, q.db, q.dt, prec, eh, q.d2, nxh, ny, 1, stream));
-            else
+            } else {
                 PDO_TRY(pdo_pade6stagg_apply(h->dz, 12, q.b, q.t, prec, q.c, q.d2, nxh, ny, 1, stream));
+            }
         }
         PDO_TRY(pdo_pade6stagg_apply(h->dz, 14, q.db, q.dt, prec, eh, slot(HC, cC, q.k), nxh, ny, 1, stream));
         PDO_TRY(pdo_spectral_ifft(h->spC, prec, slot(HC, cC, q.k), slot(RC, rC, q.k), 0, stream));

@@ -25,8 +25,15 @@ struct RhsObject {
     void *tmp = nullptr, *div = nullptr;
     double *partial = nullptr;
     int nblocks = 1024;
+    // the nine gradient passes run on a second stream, concurrently with the
+    // filter chain; fork and join are events on the caller's stream
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     ~RhsObject()
     {
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
         for (void *p : ut) if (p) (void)hipFree(p);
         if (tmp) (void)hipFree(tmp);
         if (div) (void)hipFree(div);
@@ -61,6 +68,9 @@ int pdo_rhs_init(pdo_rhs *h, int n, double L, int prec)
     if (e == hipSuccess) e = hipMalloc(&o->tmp, bytes);
     if (e == hipSuccess) e = hipMalloc(&o->div, bytes);
     if (e == hipSuccess) e = hipMalloc(&o->partial, o->nblocks * sizeof(double));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&o->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&o->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&o->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) { delete o; return hip_fail(e, "hipMalloc(rhs work buffers)"); }
     *h = o;
     return 0;
@@ -77,13 +87,21 @@ int pdo_rhs_eval(pdo_rhs h, const void *u, const void *v, const void *w, void *g
     void *uoi[3] = {uo, vo, wo};
     int r;
     // derivative / filter along direction d of a field (n, n, n)
-    auto op = [&](OpObject *o, int kind, int d, const void *f, void *out, int epi, const void *aux) {
-        return apply(o, kind, 0, 0, d, h->prec, f, out, d == 3 ? nn : n, d == 3 ? 1 : n, st, epi, aux);
+    auto op_on = [&](hipStream_t s, OpObject *o, int kind, int d, const void *f, void *out, int epi,
+                     const void *aux) {
+        return apply(o, kind, 0, 0, d, h->prec, f, out, d == 3 ? nn : n, d == 3 ? 1 : n, s, epi, aux);
     };
-    // 9 velocity-gradient components
+    auto op = [&](OpObject *o, int kind, int d, const void *f, void *out, int epi, const void *aux) {
+        return op_on(st, o, kind, d, f, out, epi, aux);
+    };
+    // 9 velocity-gradient components, on the side stream (they only read u_i)
+    hipError_t e = hipEventRecord(h->ev_fork, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(h->side, h->ev_fork, 0);
+    if (e != hipSuccess) return hip_fail(e, "stream fork");
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++)
-            if ((r = op(&h->d1, KIND_D1, j + 1, ui[i], h->field(grad, 3 * i + j), 0, nullptr))) return r;
+            if ((r = op_on(h->side, &h->d1, KIND_D1, j + 1, ui[i], h->field(grad, 3 * i + j), 0, nullptr))) return r;
+    if ((e = hipEventRecord(h->ev_join, h->side)) != hipSuccess) return hip_fail(e, "stream join");
     // filtered velocity ut_i = F_z F_y F_x u_i (uo_i is the x-pass scratch)
     for (int i = 0; i < 3; i++) {
         if ((r = op(&h->fil, KIND_FILTER, 1, ui[i], uoi[i], 0, nullptr))) return r;
@@ -98,6 +116,8 @@ int pdo_rhs_eval(pdo_rhs h, const void *u, const void *v, const void *w, void *g
     if ((r = h->poisson.solve(h->prec, h->div, h->div, st))) return r;
     for (int i = 0; i < 3; i++)
         if ((r = op(&h->d1, KIND_D1, i + 1, h->div, uoi[i], 2, h->ut[i]))) return r;
+    // everything after this call on the caller's stream also follows the gradients
+    if ((e = hipStreamWaitEvent(st, h->ev_join, 0)) != hipSuccess) return hip_fail(e, "stream join");
     return 0;
 }
 

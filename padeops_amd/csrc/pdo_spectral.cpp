@@ -239,6 +239,21 @@ int pdo_spectral_wavenumbers(pdo_spectral h, double *k1, double *k2, double *k3,
     return 0;
 }
 
+int pdo_spectral_set_factors(pdo_spectral h, const double *g1, const double *g2, const double *g3)
+{
+    if (!h) { set_error("null handle"); return PDO_ERR_ARG; }
+    struct F { const double *src; std::vector<double> *host; const double *dev; size_t n; };
+    const F f[3] = {{g1, &h->g1, h->d_g1, (size_t)h->nxh}, {g2, &h->g2, h->d_g2, (size_t)h->ny},
+                    {g3, &h->g3, h->d_g3, (size_t)h->nz}};
+    for (const F &q : f) {
+        if (!q.src) continue;
+        q.host->assign(q.src, q.src + q.n);
+        hipError_t e = hipMemcpy((void *)q.dev, q.src, q.n * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpy(filter factors)");
+    }
+    return 0;
+}
+
 int pdo_spectral_destroy(pdo_spectral h)
 {
     delete h;

@@ -36,7 +36,7 @@ def _run(o, meth, op, n, flags, n1, n2, is_complex, dtype, prec, cuda, seed):
 
 
 @pytest.mark.parametrize("combo", list(PADE6))
-@pytest.mark.parametrize("n", [64, 40])
+@pytest.mark.parametrize("n", [64, 63, 40])
 def test_stagg_np_fp64(cuda, combo, n):
     import torch
     te, be, ts, bs = PADE6[combo]
