@@ -57,9 +57,11 @@ template <> struct Mode<MODE_F32R> { using TS = float; using TC = float; static 
 template <> struct Mode<MODE_F32S> { using TS = float; using TC = double; static constexpr bool REFINE = false; };
 template <> struct Mode<MODE_F32> { using TS = float; using TC = float; static constexpr bool REFINE = false; };
 
+
 template <typename T> __device__ __forceinline__ const T *tab(const ApplyArgs &A);
 template <> __device__ __forceinline__ const double *tab<double>(const ApplyArgs &A) { return A.tab_d; }
 template <> __device__ __forceinline__ const float *tab<float>(const ApplyArgs &A) { return A.tab_f; }
+
 
 // interior stencil of output row j; window index of input row (row0 + j + o)
 // is j + R + o.
@@ -128,8 +130,8 @@ __device__ __forceinline__ T rhs_row(const TW (&win)[W], int j, const T *c, cons
 #define PDO_SCHED_FENCE(j, SB) \
     if (((j) % (SB)) == (SB) - 1) __builtin_amdgcn_sched_barrier(0)
 
-template <int K, int MM, bool EXACT, typename T>
-__device__ __forceinline__ void local_solve(T (&d)[MM], const T *fac, int m)
+template <int K, int MM, bool EXACT, typename T, typename PT>
+__device__ __forceinline__ void local_solve(T (&d)[MM], PT fac, int m)
 {
 #pragma unroll
     for (int j = 1; j < MM; ++j) {
@@ -152,20 +154,73 @@ __device__ __forceinline__ void local_solve(T (&d)[MM], const T *fac, int m)
     }
 }
 
-template <int K, int MM, bool EXACT, typename T>
-__device__ __forceinline__ void apply_spikes(T (&d)[MM], const T *spk, const T (&yl)[2], const T (&yr)[2], int m)
+template <int K, int MM, bool EXACT, typename T, typename PT>
+__device__ __forceinline__ void apply_spikes(T (&d)[MM], PT spk, const T (&yl)[2], const T (&yr)[2], int m)
 {
 #pragma unroll
     for (int j = 0; j < MM; ++j) {
         if (EXACT || j < m) {
             T v = d[j];
 #pragma unroll
-            for (int q = 0; q < K; ++q) v -= spk[j * 4 + q] * yl[q] + spk[j * 4 + 2 + q] * yr[q];
+            for (int q = 0; q < K; ++q) {
+                v = fma(-spk[j * 4 + q], yl[q], v);
+                v = fma(-spk[j * 4 + 2 + q], yr[q], v);
+            }
             d[j] = v;
         }
         PDO_SCHED_FENCE(j, 8);
     }
 }
+
+// ---------------------------------------------------------------------------
+// raw buffer access: a uniform resource over a field plus a per-lane and a
+// scalar byte offset, so that rows of a strided pencil (scalar offsets
+// row * stride) cost no vector ALU work.  Byte offsets are 32-bit: the host
+// sends fields of 4 GiB or more to the general kernel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void *p)
+{
+    // stride 0, no range limit, 32-bit data format (raw dword access)
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, -1, 0x00020000);
+}
+template <typename T> __device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so);
+template <> __device__ __forceinline__ float buf_ld<float>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0));
+}
+template <> __device__ __forceinline__ double buf_ld<double>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0));
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vo, (int)so, 0);
+}
+__device__ __forceinline__ void buf_st(double v, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
+{
+    using U2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, (int)vo, (int)so, 0);
+}
+
+// strided pencils: periodic windows and the stores through scalar row
+// offsets (1) or per-row lane address arithmetic (0); tuning constants
+#ifndef PDO_STRIDED_WINDOW_BUFFER
+#define PDO_STRIDED_WINDOW_BUFFER 1
+#endif
+#ifndef PDO_STRIDED_STORE_BUFFER
+#define PDO_STRIDED_STORE_BUFFER 1
+#endif
+// largest row stride (bytes) read with scalar row offsets.  Rows farther
+// apart (the z pass: whole planes) are addressed per row, which spaces the
+// loads of a window out: issued back to back, a window's 36-40 loads to
+// rows megabytes apart ran the 512^3 cd06 z pass at 0.30 ms instead of 0.24.
+#ifndef PDO_STRIDED_WINDOW_MAXSTRIDE
+#define PDO_STRIDED_WINDOW_MAXSTRIDE 65536u
+#endif
+// workgroup-to-pencil-group interleave of the strided kernel (1: none)
+#ifndef PDO_STRIDED_SWIZZLE
+#define PDO_STRIDED_SWIZZLE 1
+#endif
 
 // ---------------------------------------------------------------------------
 // strided pencils (y, z): NPW pencils x P segments per workgroup
@@ -177,10 +232,10 @@ __device__ __forceinline__ void apply_spikes(T (&d)[MM], const T *spk, const T (
 // are uniform, and the reduced inverse is block circulant, so the interface
 // values are sums over relative segment offsets d with uniform coefficients
 // G_0[., d] and per-lane LDS addresses.
-template <int FAM, int MODE, int NPW, bool PER>
+template <int FAM, int MODE, int NPW, bool PER, typename PT>
 __device__ __forceinline__ void reduce_seg(const typename Mode<MODE>::TC *red, const ApplyArgs &A, int s, int pl,
                                            typename Mode<MODE>::TC (&yl)[2], typename Mode<MODE>::TC (&yr)[2],
-                                           const typename Mode<MODE>::TC *Gtab)
+                                           PT Gtab)
 {
     using TC = typename Mode<MODE>::TC;
     constexpr int K = Fam<FAM>::K;
@@ -189,7 +244,7 @@ __device__ __forceinline__ void reduce_seg(const typename Mode<MODE>::TC *red, c
     if constexpr (PER) {
         // only the segment offsets whose coupling is above the working
         // precision are summed: dd in [-D, D] (all P when 2D+1 >= P)
-        const TC *G = Gtab;
+        const PT G = Gtab;
         const int D = (sizeof(TC) == 8) ? A.D64 : A.D32;
         const int nd = (2 * D + 1 < P) ? 2 * D + 1 : P;
         const int d0 = (2 * D + 1 < P) ? P - D : 0;
@@ -224,56 +279,30 @@ __device__ __forceinline__ void reduce_seg(const typename Mode<MODE>::TC *red, c
     }
 }
 
-template <int FAM, int MODE, int NPW, bool PER, bool EPI>
-__global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
+// the solve of the strided kernel: RHS from this lane's window, local solve,
+// interface exchange through `red` (LDS), spikes, and the FP64-residual
+// refinement; leaves this lane's 32 output rows in d
+template <int FAM, int MODE, int NPW, bool PER>
+__device__ __forceinline__ void strided_compute(const ApplyArgs &A, const typename Mode<MODE>::TS (&win)[MAXM + 2 * Fam<FAM>::R],
+                                                typename Mode<MODE>::TC *red, int s, int pl, int ty,
+                                                const typename Mode<MODE>::TC *fac, const typename Mode<MODE>::TC *spk,
+                                                typename Mode<MODE>::TC (&d)[MAXM])
 {
-    using TS = typename Mode<MODE>::TS;
     using TC = typename Mode<MODE>::TC;
     using F = Fam<FAM>;
-    constexpr int K = F::K, R = F::R, MM = MAXM, W = MM + 2 * R;
+    constexpr int K = F::K, MM = MAXM;
     constexpr bool REFINE = Mode<MODE>::REFINE;
-    static_assert(NPW == 64 || PER, "two segments per wave need uniform (periodic) tables");
     using TR = typename std::conditional<REFINE, double, TC>::type;  // RHS arithmetic
     const int m = MM, P = A.P;
-    const int lane = threadIdx.x & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int pl = lane % NPW;
-    const int s = (NPW == 64) ? wv : wv * 2 + (lane >> 5);
-    const long a = (long)blockIdx.x * NPW + pl;
-    const bool act = a < A.na;
-    const int ty = PER ? 1 : (P == 1 ? 3 : (s == 0 ? 0 : (s == P - 1 ? 2 : 1)));
-
-    // element offsets fit 32 bits (the host routes larger fields elsewhere):
-    // uniform base + per-lane 32-bit byte offset
-    const char *__restrict__ fb = (const char *)A.f;
-    const unsigned pin = (unsigned)((act ? a : 0) + (long)blockIdx.y * A.sb_in);
-    const unsigned se = (unsigned)A.se_in;
-    auto ldf = [&](unsigned e) { return *(const TS *)(fb + (size_t)(e * (unsigned)sizeof(TS))); };
-    TS win[W];
-    const int r0 = s * m - R;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-        int r = r0 + w;
-        bool ok = true;
-        if (PER) r = r < 0 ? r + A.n_sys : (r >= A.n_read ? r - A.n_sys : r);
-        else ok = (r >= 0) && (r < A.n_in);
-        win[w] = ok ? ldf(pin + (unsigned)r * se) : TS(0);
-    }
-
     TR cr[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) cr[q] = (TR)A.c[q];
-    TC d[MM];
 #pragma unroll
     for (int j = 0; j < MM; ++j) {
         if (PER) d[j] = (TC)stencil_row<FAM, TR>(win, j, cr);
         else d[j] = (TC)rhs_row<FAM, TR>(win, j, cr, A, ty, m);
     }
 
-    extern __shared__ unsigned char smem[];
-    TC *red = (TC *)smem;
-    const TC *fac = tab<TC>(A) + A.o_fac + ty * m * 5;
-    const TC *spk = tab<TC>(A) + A.o_spk + ty * m * 4;
     const TC *Gs = tab<TC>(A) + A.o_G;
     TC yl[2], yr[2];
 
@@ -339,18 +368,25 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
         for (int j = 0; j < MM; ++j) d[j] += e[j];
     }
 
-    if (!act) return;
-    char *ob = (char *)A.out;
-    const char *abase = (const char *)A.aux;
+}
+
+// rows of this lane's segment to out (strided by se_out), optionally
+// out = aux +- d
+template <int MODE, bool EPI>
+__device__ __forceinline__ void strided_store(const ApplyArgs &A, const typename Mode<MODE>::TC (&d)[MAXM], long a,
+                                              int s)
+{
+    using TS = typename Mode<MODE>::TS;
+    using TC = typename Mode<MODE>::TC;
+    constexpr int MM = MAXM;
+    const int m = MM;
+    constexpr unsigned SZ = sizeof(TS);
+    const __amdgpu_buffer_rsrc_t ro = field_rsrc(A.out);
+    const __amdgpu_buffer_rsrc_t ra = field_rsrc(A.aux ? A.aux : A.out);
     const unsigned po = (unsigned)(a + (long)blockIdx.y * A.sb_out);
     const unsigned so = (unsigned)A.se_out;
-    struct RowPtr {
-        char *b;
-        unsigned p;
-        __device__ TS &operator[](unsigned e) const { return *(TS *)(b + (size_t)((p + e) * (unsigned)sizeof(TS))); }
-    };
-    const RowPtr op{ob, po};
-    const RowPtr ap{(char *)abase, po};
+    const unsigned vo = (po + (unsigned)(s * m) * so) * SZ;  // row 0 of this segment
+    const unsigned ss = so * SZ;
     if (EPI) {
         // operand rows are read in chunks before the chunk's output rows are
         // written (the operand may be the output itself), so each chunk's
@@ -360,23 +396,97 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
         for (int j0 = 0; j0 < MM; j0 += CH) {
             TS av[CH];
 #pragma unroll
-            for (int j = 0; j < CH; ++j) av[j] = ap[(unsigned)(s * m + j0 + j) * so];
+            for (int j = 0; j < CH; ++j) av[j] = buf_ld<TS>(ra, vo, (unsigned)(j0 + j) * ss);
 #pragma unroll
             for (int j = 0; j < CH; ++j) {
                 const TC v = A.epi == 1 ? (TC)av[j] + d[j0 + j] : (TC)av[j] - d[j0 + j];
-                op[(unsigned)(s * m + j0 + j) * so] = (TS)v;
+                buf_st((TS)v, ro, vo, (unsigned)(j0 + j) * ss);
             }
         }
-    } else {
+    } else if (PDO_STRIDED_STORE_BUFFER) {
 #pragma unroll
-        for (int j = 0; j < MM; ++j) op[(unsigned)(s * m + j) * so] = (TS)d[j];
+        for (int j = 0; j < MM; ++j) buf_st((TS)d[j], ro, vo, (unsigned)j * ss);
+    } else {
+        TS *o = (TS *)A.out;
+#pragma unroll
+        for (int j = 0; j < MM; ++j) o[po + (unsigned)(s * m + j) * so] = (TS)d[j];
     }
     if (A.n_out > A.n_sys && s == 0) {
-        const unsigned idx = (unsigned)A.n_sys * so;
+        // periodic edge layouts repeat row 0 at row n_sys
+        const unsigned off = (unsigned)A.n_sys * ss;
         TC v = d[0];
-        if (EPI) v = A.epi == 1 ? (TC)ap[idx] + v : (TC)ap[idx] - v;
-        op[idx] = (TS)v;
+        if (EPI) {
+            const TC av = (TC)buf_ld<TS>(ra, vo, off);
+            v = A.epi == 1 ? av + v : av - v;
+        }
+        buf_st((TS)v, ro, vo, off);
     }
+}
+
+template <int FAM, int MODE, int NPW, bool PER, bool EPI>
+__global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
+{
+    using TS = typename Mode<MODE>::TS;
+    using TC = typename Mode<MODE>::TC;
+    using F = Fam<FAM>;
+    constexpr int K = F::K, R = F::R, MM = MAXM, W = MM + 2 * R;
+    constexpr bool REFINE = Mode<MODE>::REFINE;
+    static_assert(NPW == 64 || PER, "two segments per wave need uniform (periodic) tables");
+    using TR = typename std::conditional<REFINE, double, TC>::type;  // RHS arithmetic
+    const int m = MM, P = A.P;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int pl = lane % NPW;
+    const int s = (NPW == 64) ? wv : wv * 2 + (lane >> 5);
+    // consecutive workgroups take pencil groups gridDim.x / SWZ apart, so the
+    // workgroups in flight spread over the plane rather than one strip
+    unsigned bx = blockIdx.x;
+    if (PDO_STRIDED_SWIZZLE > 1 && gridDim.x % PDO_STRIDED_SWIZZLE == 0)
+        bx = (bx % PDO_STRIDED_SWIZZLE) * (gridDim.x / PDO_STRIDED_SWIZZLE) + bx / PDO_STRIDED_SWIZZLE;
+    const long a = (long)bx * NPW + pl;
+    const bool act = a < A.na;
+    const int ty = PER ? 1 : (P == 1 ? 3 : (s == 0 ? 0 : (s == P - 1 ? 2 : 1)));
+
+    // byte offsets fit 32 bits (the host sends larger fields elsewhere)
+    const __amdgpu_buffer_rsrc_t rs = field_rsrc(A.f);
+    const unsigned pin = (unsigned)((act ? a : 0) + (long)blockIdx.y * A.sb_in);
+    const unsigned se = (unsigned)A.se_in;
+    constexpr unsigned SZ = sizeof(TS);
+    TS win[W];
+    if (PDO_STRIDED_WINDOW_BUFFER && PER && A.n_read == A.n_sys && se * SZ <= PDO_STRIDED_WINDOW_MAXSTRIDE) {
+        // rows [s*m - R, s*m + m + R) of a periodic pencil: the halos wrap
+        // as whole blocks, so each region has one per-lane base (already
+        // wrapped) and row offsets that are scalar
+        const unsigned ss = se * SZ;
+        const unsigned vl = (pin + (unsigned)((s == 0 ? A.n_sys : s * m) - R) * se) * SZ;
+        const unsigned vm = (pin + (unsigned)(s * m) * se) * SZ;
+        const unsigned vr = (pin + (unsigned)(s == P - 1 ? 0 : (s + 1) * m) * se) * SZ;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            if (w < R) win[w] = buf_ld<TS>(rs, vl, (unsigned)w * ss);
+            else if (w < R + MM) win[w] = buf_ld<TS>(rs, vm, (unsigned)(w - R) * ss);
+            else win[w] = buf_ld<TS>(rs, vr, (unsigned)(w - R - MM) * ss);
+        }
+    } else {
+        const int r0 = s * m - R;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            int r = r0 + w;
+            bool ok = true;
+            if (PER) r = r < 0 ? r + A.n_sys : (r >= A.n_read ? r - A.n_sys : r);
+            else ok = (r >= 0) && (r < A.n_in);
+            win[w] = ok ? buf_ld<TS>(rs, (pin + (unsigned)(ok ? r : 0) * se) * SZ, 0u) : TS(0);
+        }
+    }
+
+    extern __shared__ unsigned char smem[];
+    TC *red = (TC *)smem;
+    const TC *fac = tab<TC>(A) + A.o_fac + ty * m * 5;
+    const TC *spk = tab<TC>(A) + A.o_spk + ty * m * 4;
+    TC d[MM];
+    strided_compute<FAM, MODE, NPW, PER>(A, win, red, s, pl, ty, fac, spk, d);
+    if (!act) return;
+    strided_store<MODE, EPI>(A, d, a, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -384,67 +494,46 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int padrow(int r) { return r + (r >> 5); }  // one pad element per 32 rows
 
-template <int FAM, int MODE, bool EPI>
-__global__ void __launch_bounds__(256) k_xdir(ApplyArgs A, int TP, int LD)
+// the compute part of the x kernel: window from the LDS tile, RHS, local
+// solve, interface exchange through `red`, spikes, and the FP64-residual
+// refinement; leaves this lane's 32 output rows in d
+template <int FAM, int MODE>
+__device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename Mode<MODE>::TS *tile,
+                                             typename Mode<MODE>::TC *red, int TP, int LD, int tpa,
+                                             const typename Mode<MODE>::TC *fac, const typename Mode<MODE>::TC *spk,
+                                             const typename Mode<MODE>::TC *G, int tid,
+                                             typename Mode<MODE>::TC (&d)[MAXM])
 {
-    // periodic pencils only: every segment has the same factors (type 1)
-    extern __shared__ unsigned char smem[];
     constexpr bool PER = true;
     using TS = typename Mode<MODE>::TS;
     using TC = typename Mode<MODE>::TC;
     using F = Fam<FAM>;
     constexpr int K = F::K, R = F::R, MM = MAXM, W = MM + 2 * R;
     constexpr bool REFINE = Mode<MODE>::REFINE;
-    using TR = typename std::conditional<REFINE, double, TC>::type;
+    using TR = typename std::conditional<REFINE, double, TC>::type;  // RHS arithmetic
     const int m = MM, P = A.P, n = A.n_sys;
-    const int tid = threadIdx.x;
     const int s = tid % P, p = tid / P;
-    const long pen0 = (long)blockIdx.x * TP;
-    const long npen = A.na * A.nb;
-    const int tpa = (int)((npen - pen0) < TP ? (npen - pen0) : TP);  // pencils of this workgroup
-    TS *tile = (TS *)smem;
-    TC *red = (TC *)(smem + (size_t)TP * LD * sizeof(TS));
-
-    // coalesced staging of tpa contiguous pencils: 16-byte loads, pencil and
-    // row from shifts (n_in is 32 * 2^k on this path)
-    using V = typename std::conditional<sizeof(TS) == 8, double2, float4>::type;
-    constexpr int VW = 16 / sizeof(TS);
-    const int lgn = A.lgn;
-    const TS *__restrict__ f = (const TS *)A.f + pen0 * (long)A.n_in;
-    const int nvec = (tpa * A.n_in) / VW;
-#pragma unroll 8
-    for (int v = tid; v < nvec; v += 256) {
-        const V x = reinterpret_cast<const V *>(f)[v];
-        const int e = v * VW, pp = e >> lgn, r = e & (A.n_in - 1);
-        TS *dst = tile + pp * LD + padrow(r);
-        const TS *xs = reinterpret_cast<const TS *>(&x);
-#pragma unroll
-        for (int i = 0; i < VW; ++i) dst[i] = xs[i];
-    }
-    __syncthreads();
-
     const bool act = p < tpa;
     const int pc = act ? p : 0;
     const int ty = PER ? 1 : (P == 1 ? 3 : (s == 0 ? 0 : (s == P - 1 ? 2 : 1)));
+    // window rows [s*m - R, s*m + m + R) of the periodic pencil; with the
+    // padded pitch row r sits at r + r/32, so segment s starts at 33 s and
+    // the halos wrap as whole blocks: three per-lane bases, constant offsets
     TS win[W];
-    const int r0 = s * m - R;
+    {
+        const int P33 = n + (n >> 5);
+        const TS *tm = tile + pc * LD + s * 33;
+        const TS *tl = tile + pc * LD + (s == 0 ? P33 : s * 33) - R - 1;
+        const TS *tr = tile + pc * LD + (s == P - 1 ? 0 : (s + 1) * 33);
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-        int r = r0 + w;
-        bool ok = true;
-        if (PER) r = r < 0 ? r + n : (r >= n ? r - n : r);
-        else ok = (r >= 0) && (r < A.n_in);
-        win[w] = ok ? tile[pc * LD + padrow(ok ? r : 0)] : TS(0);
+        for (int w = 0; w < W; ++w) win[w] = w < R ? tl[w] : (w < R + MM ? tm[w - R] : tr[w - R - MM]);
     }
     TR cr[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) cr[q] = (TR)A.c[q];
-    TC d[MM];
 #pragma unroll
     for (int j = 0; j < MM; ++j) d[j] = (TC)rhs_row<FAM, TR>(win, j, cr, A, ty, m);
 
-    const TC *fac = tab<TC>(A) + A.o_fac + ty * m * 5;
-    const TC *spk = tab<TC>(A) + A.o_spk + ty * m * 4;
     const int NR = 2 * K * P;
     auto exchange = [&](TC *rb, TC (&v)[MM], TC (&yl)[2], TC (&yr)[2]) {
 #pragma unroll
@@ -462,7 +551,6 @@ __global__ void __launch_bounds__(256) k_xdir(ApplyArgs A, int TP, int LD)
             dd -= (dd >= P) ? P : 0;
             // periodic systems are block circulant: G_s[.][t] = G_0[.][t - s]
             const int t = PER ? (s + dd < P ? s + dd : s + dd - P) : dd;
-            const TC *G = tab<TC>(A) + A.o_G + (PER ? 0 : (size_t)s * 2 * K * NR);
             const int col = PER ? dd : t;
 #pragma unroll
             for (int r = 0; r < 2 * K; ++r) {
@@ -518,6 +606,26 @@ __global__ void __launch_bounds__(256) k_xdir(ApplyArgs A, int TP, int LD)
 #pragma unroll
         for (int j = 0; j < MM; ++j) d[j] += e[j];
     }
+}
+
+// write this lane's rows d through the LDS tile to coalesced 16-byte
+// stores of A.out (optionally out = aux +- result); all threads take part
+template <int MODE, bool EPI>
+__device__ __forceinline__ void xdir_store(const ApplyArgs &A, typename Mode<MODE>::TS *tile,
+                                           const typename Mode<MODE>::TC (&d)[MAXM], int TP, int LD, int tpa)
+{
+    using TS = typename Mode<MODE>::TS;
+    using TC = typename Mode<MODE>::TC;
+    using V = typename std::conditional<sizeof(TS) == 8, double2, float4>::type;
+    constexpr int VW = 16 / sizeof(TS);
+    constexpr int MM = MAXM;
+    const int m = MM, P = A.P;
+    const int tid = threadIdx.x;
+    const int s = tid % P, p = tid / P;
+    const bool act = p < tpa;
+    const long pen0 = (long)blockIdx.x * TP;
+    const int lgn = A.lgn;
+    const int nvec = (tpa * A.n_in) / VW;
     __syncthreads();  // every window read of the tile is done
     if (act) {
 #pragma unroll
@@ -557,6 +665,54 @@ __global__ void __launch_bounds__(256) k_xdir(ApplyArgs A, int TP, int LD)
         }
         reinterpret_cast<V *>(out)[v] = x;
     }
+}
+
+template <int FAM, int MODE, bool EPI>
+__global__ void __launch_bounds__(256) k_xdir(ApplyArgs A, int TP, int LD)
+{
+    // periodic pencils only: every segment has the same factors (type 1)
+    extern __shared__ unsigned char smem[];
+    constexpr bool PER = true;
+    using TS = typename Mode<MODE>::TS;
+    using TC = typename Mode<MODE>::TC;
+    using F = Fam<FAM>;
+    constexpr int K = F::K, R = F::R, MM = MAXM, W = MM + 2 * R;
+    constexpr bool REFINE = Mode<MODE>::REFINE;
+    using TR = typename std::conditional<REFINE, double, TC>::type;
+    const int m = MM, P = A.P, n = A.n_sys;
+    const int tid = threadIdx.x;
+    const int s = tid % P, p = tid / P;
+    const long pen0 = (long)blockIdx.x * TP;
+    const long npen = A.na * A.nb;
+    const int tpa = (int)((npen - pen0) < TP ? (npen - pen0) : TP);  // pencils of this workgroup
+    TS *tile = (TS *)smem;
+    TC *red = (TC *)(smem + (size_t)TP * LD * sizeof(TS));
+
+    // coalesced staging of tpa contiguous pencils: 16-byte loads, pencil and
+    // row from shifts (n_in is 32 * 2^k on this path)
+    using V = typename std::conditional<sizeof(TS) == 8, double2, float4>::type;
+    constexpr int VW = 16 / sizeof(TS);
+    const int lgn = A.lgn;
+    const TS *__restrict__ f = (const TS *)A.f + pen0 * (long)A.n_in;
+    const int nvec = (tpa * A.n_in) / VW;
+#pragma unroll 8
+    for (int v = tid; v < nvec; v += 256) {
+        const V x = reinterpret_cast<const V *>(f)[v];
+        const int e = v * VW, pp = e >> lgn, r = e & (A.n_in - 1);
+        TS *dst = tile + pp * LD + padrow(r);
+        const TS *xs = reinterpret_cast<const TS *>(&x);
+#pragma unroll
+        for (int i = 0; i < VW; ++i) dst[i] = xs[i];
+    }
+    __syncthreads();
+
+    const bool act = p < tpa;
+    TC d[MM];
+    const TC *tb = tab<TC>(A);
+    xdir_compute<FAM, MODE>(A, tile, red, TP, LD, tpa, tb + A.o_fac + MM * 5, tb + A.o_spk + MM * 4, tb + A.o_G,
+                            (int)threadIdx.x, d);
+    xdir_store<MODE, EPI>(A, tile, d, TP, LD, tpa);
+
 }
 
 // ---------------------------------------------------------------------------

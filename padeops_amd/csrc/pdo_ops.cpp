@@ -2,6 +2,7 @@
 #include "pdo_ops.hpp"
 
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -114,23 +115,26 @@ int prec_mode(int prec)
     return -1;
 }
 
-int apply(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, const void *f, void *out, long na, long nb,
-          hipStream_t stream, int epi, const void *aux, int force_shape)
+// Validate one apply call and fill its launch arguments; *empty is set when
+// there is nothing to do (na or nb zero).
+static int prepare(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, const void *f, void *out, long na,
+                   long nb, int epi, const void *aux, int force_shape, ApplyArgs &A, int &shape, int &mode,
+                   int &family, double *&scratch, bool *empty)
 {
+    *empty = false;
     if (!op || !f || !out || na < 0 || nb < 0 || dir < 1 || dir > 3) {
         set_error("invalid argument");
         return PDO_ERR_ARG;
     }
-    const int mode = prec_mode(prec);
+    mode = prec_mode(prec);
     if (mode < 0) { set_error("unknown precision mode"); return PDO_ERR_ARG; }
     if (epi && !aux) { set_error("epilogue without operand"); return PDO_ERR_ARG; }
-    if (na == 0 || nb == 0) return 0;
+    if (na == 0 || nb == 0) { *empty = true; return 0; }
     Variant *v = nullptr;
     int st = op->variant(kind, bc1, bcn, &v);
     if (st) return st;
     const SystemDesc &s = v->sys;
     long A_na, A_nb;
-    int shape;
     if (dir == 1) { A_na = na; A_nb = nb; shape = SHAPE_X; }
     else if (dir == 2) { A_na = na; A_nb = nb; shape = SHAPE_STRIDED; }
     else { A_na = na * nb; A_nb = 1; shape = SHAPE_STRIDED; }
@@ -144,6 +148,10 @@ int apply(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, const voi
     if (v->fast.tab.general || (shape == SHAPE_X && (!s.periodic || s.family >= FAM_STG_E2C_A)) ||
         force_shape == SHAPE_GENERAL)
         shape = SHAPE_GENERAL;
+    // the strided kernel addresses a field with 32-bit byte offsets
+    const size_t word = (mode == MODE_F64) ? 8 : 4;
+    const size_t nmax = (size_t)std::max(std::max(s.n_in, s.n_out), s.n_sys + 1);
+    if (shape == SHAPE_STRIDED && (size_t)na * (size_t)nb * nmax * word >= ((size_t)1 << 32)) shape = SHAPE_GENERAL;
     TabSet *ts = &v->fast;
     if (shape == SHAPE_GENERAL && !v->fast.tab.general) {
         std::lock_guard<std::mutex> lk(op->mu);
@@ -154,8 +162,8 @@ int apply(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, const voi
         ts = &v->gen;
     }
     const SpikeTables &t = ts->tab;
-    ApplyArgs A;
     std::memset(&A, 0, sizeof(A));
+    family = s.family;
     A.f = f; A.out = out; A.aux = aux; A.epi = epi;
     A.periodic = s.periodic ? 1 : 0;
     A.n_sys = s.n_sys; A.n_in = s.n_in; A.n_out = s.n_out;
@@ -182,7 +190,7 @@ int apply(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, const voi
         A.sa_in = 1; A.sb_in = 0; A.se_in = na * nb;
         A.sa_out = 1; A.sb_out = 0; A.se_out = na * nb;
     }
-    double *scratch = nullptr;
+    scratch = nullptr;
     if (shape == SHAPE_GENERAL) {
         const size_t need = (size_t)s.n_sys * (size_t)(A.na * A.nb);
         std::lock_guard<std::mutex> lk(op->mu);
@@ -196,10 +204,24 @@ int apply(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, const voi
         }
         scratch = v->scratch;
     }
-    hipError_t e = launch_compact(s.family, shape, mode, A, scratch, stream);
+    return 0;
+}
+
+int apply(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, const void *f, void *out, long na, long nb,
+          hipStream_t stream, int epi, const void *aux, int force_shape)
+{
+    ApplyArgs A;
+    int shape = 0, mode = 0, family = 0;
+    double *scratch = nullptr;
+    bool empty = false;
+    int st = prepare(op, kind, bc1, bcn, dir, prec, f, out, na, nb, epi, aux, force_shape, A, shape, mode, family,
+                     scratch, &empty);
+    if (st || empty) return st;
+    hipError_t e = launch_compact(family, shape, mode, A, scratch, stream);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return 0;
 }
+
 
 }  // namespace pdo
 

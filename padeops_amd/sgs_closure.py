@@ -132,23 +132,47 @@ def stress_metrics(pred, T):
     return float(np.mean(cors)), err
 
 
-def train(net, X, T, steps, batch=65536, lr=2e-3, seed=0, stream=None):
-    """Adam on random batches of (X, T); returns the loss of the last batch."""
+def gradient_model(X):
+    """The gradient (Clark) model tau_ij = Delta^2 / 12 d_k bar(u_i) d_k bar(u_j),
+    the leading term of the Gaussian filter's exact stress, deviatoric, in the
+    units of T: X_ik X_jk / 12.  A reference point for the a priori scores."""
     torch = _torch()
+    g = X[:, :9].double().reshape(-1, 3, 3)
+    gg = torch.einsum("nik,njk->nij", g, g) / 12.0
+    tr = (gg[:, 0, 0] + gg[:, 1, 1] + gg[:, 2, 2]) / 3.0
+    out = torch.stack([gg[:, i, j] for i, j in IDX6], dim=1)
+    for q in (0, 3, 5):
+        out[:, q] -= tr
+    return out.float()
+
+
+def train(net, X, T, steps, batch=65536, lr=2e-3, seed=0, stream=None, tscale=None):
+    """Adam on random batches of (X, T); returns the loss of the last batch.
+
+    The network learns T / tscale (tscale defaults to the rms of T): the
+    subgrid stress in units of U^2 is O(1e-2), and an O(1) target keeps the
+    output layer at its initial scale and the BF16 upstream gradients well
+    resolved.  predict() multiplies by the same factor (kept on the net)."""
+    torch = _torch()
+    if tscale is None:
+        tscale = float(torch.sqrt((T.double() ** 2).mean()))
+    net.tscale = tscale
+    Tn = (T / tscale).contiguous()
     g = torch.Generator(device=X.device)
     g.manual_seed(seed)
     Y = torch.empty((batch, 6), dtype=torch.float32, device=X.device)
     for s in range(steps):
         idx = torch.randint(0, X.shape[0], (batch,), generator=g, device=X.device)
-        xb, tb = X[idx].contiguous(), T[idx].contiguous()
+        xb, tb = X[idx].contiguous(), Tn[idx].contiguous()
         lr_s = lr * 0.5 * (1 + math.cos(math.pi * s / steps))
         net.train_step(xb, tb, lr=lr_s, Y=Y, stream=stream)
     return float(((Y - tb) ** 2).mean())
 
 
 def predict(net, X, chunk=1 << 20, stream=None):
+    """Network stress for the rows of X, in the units of T."""
     torch = _torch()
     out = torch.empty((X.shape[0], 6), dtype=torch.float32, device=X.device)
     for a in range(0, X.shape[0], chunk):
         net.forward(X[a:a + chunk].contiguous(), out[a:a + chunk], stream=stream)
-    return out
+    return out * getattr(net, "tscale", 1.0)

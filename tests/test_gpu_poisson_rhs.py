@@ -96,7 +96,7 @@ def _rhs_gpu(n, u, v, w, prec, cuda):
     return rhs, g, o, out
 
 
-@pytest.mark.parametrize("n", [32, 64])
+@pytest.mark.parametrize("n", [32, 64, 128])
 def test_full_rhs_fp64(cuda, n):
     u, v, w = isotropic_velocity(n, k0=4.0, seed=42)
     rhs, g, o, out_t = _rhs_gpu(n, u, v, w, "f64", cuda)
@@ -115,11 +115,17 @@ def test_full_rhs_fp64(cuda, n):
     assert maxdiv < 1e-12
 
 
-def test_full_rhs_fp32_refine_error(cuda):
-    n = 64
+# n = 128 has four segments per pencil.  Plain FP32 solves of the cf90 system
+# (its matrix symbol 1 - 2 alpha + 2 beta = 1.3e-3 at the Nyquist wavenumber,
+# condition number ~2e3) leave ~5e-5 in the filtered and projected fields,
+# which is what the FP64-residual refinement removes.
+@pytest.mark.parametrize("prec,n,bar", [("f32_refine", 64, 1e-6), ("f32_refine", 128, 1e-6),
+                                        ("f32_store", 64, 1e-6), ("f32", 64, 2e-4)])
+def test_full_rhs_fp32_error(cuda, prec, n, bar):
     u, v, w = isotropic_velocity(n, k0=4.0, seed=7)
-    _, g, o, _ = _rhs_gpu(n, u, v, w, "f32_refine", cuda)
+    _, g, o, _ = _rhs_gpu(n, u, v, w, prec, cuda)
     grad_ref, uo, vo, wo, _ = O.full_rhs(u, v, w)
     errs = [rel_l2(g[q], grad_ref[q]) for q in range(9)] + [rel_l2(a, b) for a, b in zip(o, (uo, vo, wo))]
-    print("fp32+refine rel errors", errs)
-    assert max(errs) <= 1e-6
+    print(prec, n, "rel errors", errs)
+    assert max(errs) <= bar
+
