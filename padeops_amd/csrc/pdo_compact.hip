@@ -489,6 +489,14 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
     strided_store<MODE, EPI>(A, d, a, s);
 }
 
+// exchange buffers of the refined x kernel: 3 (one per exchange) or 1
+// (shared, two more barriers; 8 KiB less LDS per workgroup for the
+// pentadiagonal systems, which lets a fourth workgroup share a CU)
+#ifndef PDO_XDIR_RED_SLOTS
+#define PDO_XDIR_RED_SLOTS 1
+#endif
+static_assert(PDO_XDIR_RED_SLOTS == 1 || PDO_XDIR_RED_SLOTS == 3, "1 or 3 exchange buffers");
+
 // ---------------------------------------------------------------------------
 // contiguous pencils (x): TP pencils staged in LDS, lanes = (pencil, segment)
 // ---------------------------------------------------------------------------
@@ -570,7 +578,8 @@ __device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename 
     apply_spikes<K, MM, true>(d, spk, yl, yr, m);
     if constexpr (REFINE) {
         // neighbour rows of the residual from the values the neighbours hold
-        TC *rb3 = red + (size_t)2 * TP * P * 2 * K;
+        TC *rb3 = red + (size_t)(PDO_XDIR_RED_SLOTS == 3 ? 2 : 0) * TP * P * 2 * K;
+        if (PDO_XDIR_RED_SLOTS == 1) __syncthreads();  // the exchange's reads of the slot are done
         (void)ty;
 #pragma unroll
         for (int q = 0; q < K; ++q) {
@@ -601,7 +610,8 @@ __device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename 
         }
         local_solve<K, MM, true>(e, fac, m);
         TC el[2], er[2];
-        exchange(red + (size_t)TP * P * 2 * K, e, el, er);
+        if (PDO_XDIR_RED_SLOTS == 1) __syncthreads();  // the neighbour reads of the slot are done
+        exchange(red + (size_t)(PDO_XDIR_RED_SLOTS == 3 ? 1 : 0) * TP * P * 2 * K, e, el, er);
         apply_spikes<K, MM, true>(e, spk, el, er, m);
 #pragma unroll
         for (int j = 0; j < MM; ++j) d[j] += e[j];
@@ -873,7 +883,8 @@ static hipError_t launch_x_t(const ApplyArgs &A, hipStream_t st)
     int TP = 256 / P;
     const int LD = A.n_in + (A.n_in >> 5) + 1;
     auto lds_of = [&](int tp) {
-        return (size_t)tp * LD * sizeof(TS) + (size_t)tp * P * 2 * K * sizeof(TC) * (Mode<MODE>::REFINE ? 3 : 1);
+        return (size_t)tp * LD * sizeof(TS) +
+               (size_t)tp * P * 2 * K * sizeof(TC) * (Mode<MODE>::REFINE ? PDO_XDIR_RED_SLOTS : 1);
     };
     while (TP > 1 && lds_of(TP) > 80 * 1024) TP >>= 1;
     if (TP * P != 256) return hipErrorInvalidValue;
