@@ -217,6 +217,15 @@ __device__ __forceinline__ void buf_st(double v, __amdgpu_buffer_rsrc_t r, unsig
 #ifndef PDO_STRIDED_WINDOW_MAXSTRIDE
 #define PDO_STRIDED_WINDOW_MAXSTRIDE 65536u
 #endif
+// epilogue operand rows: rows per load batch, and the largest stride read
+// with scalar row offsets (512^3 z pass with an operand: 0.41 ms with
+// scalar row offsets, 0.36 ms with per-row addresses)
+#ifndef PDO_STRIDED_EPI_CHUNK
+#define PDO_STRIDED_EPI_CHUNK 16
+#endif
+#ifndef PDO_STRIDED_EPI_MAXSTRIDE
+#define PDO_STRIDED_EPI_MAXSTRIDE 65536u
+#endif
 // workgroup-to-pencil-group interleave of the strided kernel (1: none)
 #ifndef PDO_STRIDED_SWIZZLE
 #define PDO_STRIDED_SWIZZLE 1
@@ -391,12 +400,18 @@ __device__ __forceinline__ void strided_store(const ApplyArgs &A, const typename
         // operand rows are read in chunks before the chunk's output rows are
         // written (the operand may be the output itself), so each chunk's
         // loads issue together without holding all 32 rows in registers
-        constexpr int CH = sizeof(TS) == 8 ? 8 : 16;
+        constexpr int CH = sizeof(TS) == 8 ? 8 : PDO_STRIDED_EPI_CHUNK;
+        const bool far = ss > PDO_STRIDED_EPI_MAXSTRIDE;
 #pragma unroll
         for (int j0 = 0; j0 < MM; j0 += CH) {
             TS av[CH];
+            if (far) {
 #pragma unroll
-            for (int j = 0; j < CH; ++j) av[j] = buf_ld<TS>(ra, vo, (unsigned)(j0 + j) * ss);
+                for (int j = 0; j < CH; ++j) av[j] = buf_ld<TS>(ra, vo + (unsigned)(j0 + j) * ss, 0u);
+            } else {
+#pragma unroll
+                for (int j = 0; j < CH; ++j) av[j] = buf_ld<TS>(ra, vo, (unsigned)(j0 + j) * ss);
+            }
 #pragma unroll
             for (int j = 0; j < CH; ++j) {
                 const TC v = A.epi == 1 ? (TC)av[j] + d[j0 + j] : (TC)av[j] - d[j0 + j];

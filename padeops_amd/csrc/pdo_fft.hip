@@ -207,20 +207,25 @@ template <typename TC> struct CLoadF {
 template <> struct CLoad<float, float> : CLoadF<float> {};
 template <> struct CLoad<float, double> : CLoadF<double> {};
 
+// 1/x: correctly rounded in FP64; the hardware reciprocal (1 ulp) in FP32
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // ---------------------------------------------------------------------------
 // passes 2-4: complex FFT of strided lines.  Line (a, b) starts at a + b*sb
 // (complex units), element stride se; a in [0, na) is the contiguous index.
 // Tile: TX = 1 << lgtx consecutive lines; thread = (line, jb), line fastest.
 // MODE 0 forward, 1 inverse, 2 forward -> Poisson multiply -> inverse.
 // ---------------------------------------------------------------------------
-template <typename TS, typename TC, int MODE, int L>
+template <typename TS, typename TC, int MODE, int L, int TX>
 __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, long na, long sb, int se,
-                                                     const TC *__restrict__ twL, int lgtx, SpecMul pm)
+                                                     const TC *__restrict__ twL, SpecMul pm)
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
-    const int TX = 1 << lgtx;
-    const int ln = threadIdx.x & (TX - 1), jb = threadIdx.x >> lgtx;
+    // TX is a compile-time constant so that the LDS addresses of a thread's
+    // elements are one base register plus immediate offsets
+    const int ln = threadIdx.x & (TX - 1), jb = threadIdx.x / TX;
     const long a = (long)blockIdx.x * TX + ln;
     const bool act = a < na;
     const long off = 2 * ((act ? a : 0) + (long)blockIdx.y * sb);
@@ -230,7 +235,7 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
     using IO = CLoad<TS, TC>;
     auto ldg = [&](int i) { return IO::ld(lin, i * se); };
     auto stg = [&](int i, C2<TC> v) { IO::st(lout, i * se, v); };
-    auto buf = [&](int i) -> C2<TC> & { return tile[(i << lgtx) + ln]; };
+    auto buf = [&](int i) -> C2<TC> & { return tile[i * TX + ln]; };
     if constexpr (MODE == 0) {
         line_fft<TC, -1, false, L>(ldg, stg, buf, tw, jb, act);
     } else if constexpr (MODE == 1) {
@@ -238,20 +243,24 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
     } else {
         // forward into LDS, multiply by the spectral factor of (kx, ky, kz) =
         // (a, blockIdx.y, i), inverse from LDS to HBM
+        // the factor is formed in the arithmetic of the transform (an FP64
+        // division per point costs as much as the transforms in FP32)
         const int kx = (int)a, ky = (int)blockIdx.y;
-        const double fxy = act ? (pm.kind == 2 ? pm.ax[kx] * pm.ay[ky] : pm.ax[kx] + pm.ay[ky]) : 0.0;
+        const TC fxy = act ? (TC)(pm.kind == 2 ? pm.ax[kx] * pm.ay[ky] : pm.ax[kx] + pm.ay[ky]) : TC(0);
+        const TC scale = (TC)pm.scale;
         auto sts = [&](int i, C2<TC> v) {
-            double fac;
+            TC fac;
+            const TC az = (TC)pm.az[i];
             if (pm.kind == 2) {
-                fac = fxy * pm.az[i];
+                fac = fxy * az;
             } else {
-                const double ks = fxy + pm.az[i];
-                if (pm.kind == 0) fac = (kx == 0 && ky == 0 && i == 0) ? 0.0 : -1.0 / (ks + 1.0e-20);
-                else fac = (ks <= 1.0e-14) ? 0.0 : -1.0 / ks;
+                const TC ks = fxy + az;
+                if (pm.kind == 0) fac = (kx == 0 && ky == 0 && i == 0) ? TC(0) : -recip(ks + TC(1.0e-20));
+                else fac = (ks <= TC(1.0e-14)) ? TC(0) : -recip(ks);
             }
-            fac *= pm.scale;
-            v.x *= (TC)fac;
-            v.y *= (TC)fac;
+            fac *= scale;
+            v.x *= fac;
+            v.y *= fac;
             buf(i) = v;
         };
         line_fft<TC, -1, false, L>(ldg, sts, buf, tw, jb, act);
@@ -380,19 +389,25 @@ static int ilog2(int v)
 #define PDO_FFT_X_THREADS 1024
 #endif
 
+// lines per workgroup of the strided transforms: PDO_FFT_STRIDED_THREADS
+// threads (8 points each), at most 64 KiB of LDS
+template <int L, size_t CE> constexpr int strided_fft_lines()
+{
+    int tx = PDO_FFT_STRIDED_THREADS / (L / 8);
+    if (tx < 1) tx = 1;
+    while (tx > 1 && (size_t)tx * L * CE > 64 * 1024) tx >>= 1;
+    return tx;
+}
+
 template <typename TS, typename TC, int MODE, int L>
 static void launch_strided_fft(const TS *src, TS *dst, long na, long sb, long se, const TC *tw, const SpecMul &pm,
                                int nb, hipStream_t st)
 {
-    const size_t ce = 2 * sizeof(TC);
-    int TX = PDO_FFT_STRIDED_THREADS / (L / 8);
-    if (TX < 1) TX = 1;
-    while (TX > 1 && (size_t)TX * L * ce > 64 * 1024) TX >>= 1;
-    int lg = 0;
-    while ((1 << lg) < TX) lg++;
+    constexpr size_t ce = 2 * sizeof(TC);
+    constexpr int TX = strided_fft_lines<L, ce>();
     dim3 grid((unsigned)((na + TX - 1) / TX), (unsigned)nb);
-    hipLaunchKernelGGL((k_fft_strided<TS, TC, MODE, L>), grid, dim3(TX * (L / 8)), (size_t)TX * L * ce, st, src, dst,
-                       na, sb, (int)se, tw, lg, pm);
+    hipLaunchKernelGGL((k_fft_strided<TS, TC, MODE, L, TX>), grid, dim3(TX * (L / 8)), (size_t)TX * L * ce, st, src,
+                       dst, na, sb, (int)se, tw, pm);
 }
 
 template <typename TS, typename TC, int MODE>

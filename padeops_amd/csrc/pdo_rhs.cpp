@@ -12,6 +12,12 @@
 #include "pdo_ops.hpp"
 #include "pdo_poisson.hpp"
 
+// run the nine gradient passes on a second stream (1) or in order on the
+// caller's stream (0); tuning constant
+#ifndef PDO_RHS_SIDE_STREAM
+#define PDO_RHS_SIDE_STREAM 1
+#endif
+
 namespace pdo {
 
 struct RhsObject {
@@ -100,7 +106,9 @@ int pdo_rhs_eval(pdo_rhs h, const void *u, const void *v, const void *w, void *g
     if (e != hipSuccess) return hip_fail(e, "stream fork");
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++)
-            if ((r = op_on(h->side, &h->d1, KIND_D1, j + 1, ui[i], h->field(grad, 3 * i + j), 0, nullptr))) return r;
+            if ((r = op_on(PDO_RHS_SIDE_STREAM ? h->side : st, &h->d1, KIND_D1, j + 1, ui[i], h->field(grad, 3 * i + j),
+                           0, nullptr)))
+                return r;
     if ((e = hipEventRecord(h->ev_join, h->side)) != hipSuccess) return hip_fail(e, "stream join");
     // filtered velocity ut_i = F_z F_y F_x u_i (uo_i is the x-pass scratch)
     for (int i = 0; i < 3; i++) {
