@@ -68,6 +68,9 @@ typedef struct pdo_pade6stagg_s *pdo_pade6stagg;
 typedef struct pdo_duidxj_s *pdo_duidxj;
 typedef struct pdo_dynproc_s *pdo_dynproc;
 typedef struct pdo_mlp_s *pdo_mlp;
+typedef struct pdo_xfilter_s *pdo_xfilter;
+typedef struct pdo_derivatives_s *pdo_derivatives;
+typedef struct pdo_filters_s *pdo_filters;
 
 /* ---- library ------------------------------------------------------------ */
 const char *pdo_last_error(void);
@@ -321,6 +324,73 @@ int pdo_mlp_adam_step(pdo_mlp h, float lr, float b1, float b2, float eps, void *
 int pdo_mlp_train_step(pdo_mlp h, const float *X, const float *T, long B, float lr, float b1, float b2, float eps,
                        float *Y, void *stream);
 int pdo_mlp_destroy(pdo_mlp h);
+
+/* ---- explicit filters: gaussianstuff / lstsqstuff ------------------------
+ * gaussian%init(n, periodic) (gaussian.F90:74-93) and lstsq%init
+ * (lstsq.F90:74-93): symmetric 9-point filters (Gaussian of width 4 dx;
+ * least squares), with 4 one-sided closure rows per end when non-periodic.
+ * n == 1 copies; otherwise n >= 8 (PDO_ERR_ARG below).
+ * filter1/2/3 (gaussian.F90:104-573, lstsq.F90:104-523): same layouts and
+ * arguments as the cf90 filters.  Gaussian bc1/bcn: 0 one-sided rows, +1 / -1
+ * even / odd reflection about the end point; other values return 324
+ * (GracefulExit 324).  The least-squares filter ignores bc1/bcn.
+ * FP32 storage is filtered in FP64 arithmetic (all FP32 modes agree). */
+int pdo_gaussian_init(pdo_xfilter *h, int n, int periodic);
+int pdo_lstsq_init(pdo_xfilter *h, int n, int periodic);
+int pdo_xfilter_filter1(pdo_xfilter h, int prec, const void *f, void *fil, int na, int nb, int bc1, int bcn,
+                        void *stream);
+int pdo_xfilter_filter2(pdo_xfilter h, int prec, const void *f, void *fil, int na, int nb, int bc1, int bcn,
+                        void *stream);
+int pdo_xfilter_filter3(pdo_xfilter h, int prec, const void *f, void *fil, int na, int nb, int bc1, int bcn,
+                        void *stream);
+/* complex (na, nb, n), interleaved (filter3_cmplx, gaussian.F90:574-582) */
+int pdo_xfilter_filter3_cmplx(pdo_xfilter h, int prec, const void *f, void *fil, int na, int nb, int bc1, int bcn,
+                              void *stream);
+/* host only: the n x n matrix (row major) the filter applies with (bc1, bcn) */
+int pdo_xfilter_dense(pdo_xfilter h, int bc1, int bcn, double *W);
+int pdo_xfilter_destroy(pdo_xfilter h);
+
+/* ---- derivatives / filters dispatch and the operators module -------------
+ * derivatives%init (derivatives.F90:140-186,264-390) on one GPU: sizes
+ * (nx, ny, nz) are the pencil sizes of every direction.  method[d]:
+ * 1 "cd10", 2 "cd06", 3 "four" (Fourier collocation, ffts.F90: power-of-two
+ * n, x 16..1024, y/z 8..1024, else PDO_ERR_UNSUP), 4 "cheb" (PDO_ERR_UNSUP:
+ * incomplete in the reference); other values PDO_ERR_ARG.
+ * ddx..d2dz2 (derivatives.F90:447-569) on a field (nx, ny, nz): cd10 takes
+ * bc1/bcn, cd06 and four ignore them; cd06 second derivatives return
+ * PDO_ERR_UNSUP ("CD06 is incomplete", GracefulExit 21).  A Fourier y (z)
+ * derivative needs nx (nx*ny) even. */
+int pdo_derivatives_init(pdo_derivatives *h, int nx, int ny, int nz, double dx, double dy, double dz,
+                         const int *periodic, const int *method);
+int pdo_derivatives_method(pdo_derivatives h, int dir, int *method);
+int pdo_derivatives_ddx(pdo_derivatives h, int prec, const void *f, void *df, int bc1, int bcn, void *stream);
+int pdo_derivatives_ddy(pdo_derivatives h, int prec, const void *f, void *df, int bc1, int bcn, void *stream);
+int pdo_derivatives_ddz(pdo_derivatives h, int prec, const void *f, void *df, int bc1, int bcn, void *stream);
+int pdo_derivatives_d2dx2(pdo_derivatives h, int prec, const void *f, void *df, int bc1, int bcn, void *stream);
+int pdo_derivatives_d2dy2(pdo_derivatives h, int prec, const void *f, void *df, int bc1, int bcn, void *stream);
+int pdo_derivatives_d2dz2(pdo_derivatives h, int prec, const void *f, void *df, int bc1, int bcn, void *stream);
+int pdo_derivatives_destroy(pdo_derivatives h);
+/* filters%init (filters.F90:85-175): method[d] 1 "cf90", 2 "gaussian",
+ * 3 "lstsq"; filterx/y/z (filters.F90:220-269) on a field (nx, ny, nz). */
+int pdo_filters_init(pdo_filters *h, int nx, int ny, int nz, const int *periodic, const int *method);
+int pdo_filters_method(pdo_filters h, int dir, int *method);
+int pdo_filters_filterx(pdo_filters h, int prec, const void *f, void *ff, int bc1, int bcn, void *stream);
+int pdo_filters_filtery(pdo_filters h, int prec, const void *f, void *ff, int bc1, int bcn, void *stream);
+int pdo_filters_filterz(pdo_filters h, int prec, const void *f, void *ff, int bc1, int bcn, void *stream);
+int pdo_filters_destroy(pdo_filters h);
+/* operators.F90: bc = {x_bc(1:2), y_bc(1:2), z_bc(1:2)} (NULL: zeros).
+ * gradient (:18-53); curl (:55-116) into 3 consecutive fields
+ * (dw/dy - dv/dz, du/dz - dw/dx, dv/dx - du/dy); divergence (:118-156) with
+ * the bc signs reversed; filter3D (:158-223) filters arr in place in y, x,
+ * then z, numtimes times each (numtimes < 1 means 1).  The derivatives /
+ * filters object holds the work fields (allocated at the first call). */
+int pdo_operators_gradient(pdo_derivatives der, int prec, const void *f, void *dfdx, void *dfdy, void *dfdz,
+                           const int *bc, void *stream);
+int pdo_operators_curl(pdo_derivatives der, int prec, const void *u, const void *v, const void *w, void *curl,
+                       const int *bc, void *stream);
+int pdo_operators_divergence(pdo_derivatives der, int prec, const void *u, const void *v, const void *w, void *div,
+                             const int *bc, void *stream);
+int pdo_operators_filter3d(pdo_filters fil, int prec, void *arr, int numtimes, const int *bc, void *stream);
 
 /* ---- full RHS evaluation (the benchmarked composite) ----------------------
  * On a triply periodic n^3 grid of side L, for velocity (u, v, w):

@@ -220,6 +220,53 @@ class cf90(_Op):
                                            _stream(stream)))
 
 
+class _XFilter(_Op):
+    """Explicit 9-point filters; filter1/2/3 as cf90's."""
+    _init_name = None
+
+    def init(self, n, periodic):
+        self.n = n
+        return _check(getattr(lib(), self._init_name)(ctypes.byref(self._h), n, int(periodic)))
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_xfilter_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def _ap(self, name, f, df, na, nb, bc1, bcn, prec, stream):
+        return _apply(getattr(lib(), name), self._handle(), f, df, na, nb, self.n, self.n, prec, stream, bc1, bcn)
+
+    def dense_matrix(self, bc1=0, bcn=0):
+        """n x n matrix of the filter (host-side, no GPU)."""
+        import numpy as np
+        W = np.zeros((self.n, self.n))
+        _check(lib().pdo_xfilter_dense(self._handle(), bc1, bcn, W.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return W
+
+    def filter1(self, f, fil, na, nb, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_xfilter_filter1", f, fil, na, nb, bc1, bcn, prec, stream)
+
+    def filter2(self, f, fil, na, nb, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_xfilter_filter2", f, fil, na, nb, bc1, bcn, prec, stream)
+
+    def filter3(self, f, fil, na, nb, bc1=0, bcn=0, prec="f32_refine", stream=None, complex_=False):
+        name = "pdo_xfilter_filter3_cmplx" if complex_ else "pdo_xfilter_filter3"
+        k = 2 if complex_ else 1
+        _check_sizes(f, fil, self.n * k, self.n * k, na, nb)
+        return _check(getattr(lib(), name)(self._handle(), _prec(f, prec), _ptr(f), _ptr(fil), na, nb, bc1, bcn,
+                                           _stream(stream)))
+
+
+class gaussian(_XFilter):
+    """type(gaussian), src/filters/gaussian.F90: the width-4dx Gaussian filter."""
+    _init_name = "pdo_gaussian_init"
+
+
+class lstsq(_XFilter):
+    """type(lstsq), src/filters/lstsq.F90: the least-squares filter (no bc options)."""
+    _init_name = "pdo_lstsq_init"
+
+
 class cd06stagg(_Op):
     """type(cd06stagg), src/derivatives/cd06stagg.F90: init(n, dz) is the
     periodic object, init(n, dz, isTopEven, isBotEven[, isTopSided,
@@ -300,6 +347,167 @@ class PoissonPeriodic:
             self.destroy()
         except Exception:
             pass
+
+
+DER_METHOD = {"cd10": 1, "cd06": 2, "four": 3, "cheb": 4}
+FIL_METHOD = {"cf90": 1, "gaussian": 2, "lstsq": 3}
+
+
+def _ints(vals):
+    return (ctypes.c_int * len(vals))(*[int(v) for v in vals])
+
+
+def _field_check(obj, *ts):
+    n = obj.nx * obj.ny * obj.nz
+    for t in ts:
+        if t is not None and t.numel() < n:
+            raise ValueError("buffer smaller than the (nx, ny, nz) field")
+
+
+class derivatives:
+    """type(derivatives), src/derivatives/derivatives.F90 (serial init: the
+    pencil sizes of every direction are (nx, ny, nz))."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def init(self, nx, ny, nz, dx, dy, dz, periodic_x, periodic_y, periodic_z, method_x, method_y, method_z):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        meth = [DER_METHOD.get(m, 0) for m in (method_x, method_y, method_z)]
+        return _check(lib().pdo_derivatives_init(ctypes.byref(self._h), nx, ny, nz, dx, dy, dz,
+                                                 _ints([periodic_x, periodic_y, periodic_z]), _ints(meth)))
+
+    def _method(self, d):
+        m = ctypes.c_int()
+        _check(lib().pdo_derivatives_method(self._h, d, ctypes.byref(m)))
+        return {v: k for k, v in DER_METHOD.items()}[m.value]
+
+    def getMethodx(self):
+        return self._method(1)
+
+    def getMethody(self):
+        return self._method(2)
+
+    def getMethodz(self):
+        return self._method(3)
+
+    def _ap(self, name, f, df, bc1, bcn, prec, stream):
+        _field_check(self, f, df)
+        return _check(getattr(lib(), name)(self._h, _prec(f, prec), _ptr(f), _ptr(df), bc1, bcn, _stream(stream)))
+
+    def ddx(self, f, df, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_derivatives_ddx", f, df, bc1, bcn, prec, stream)
+
+    def ddy(self, f, df, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_derivatives_ddy", f, df, bc1, bcn, prec, stream)
+
+    def ddz(self, f, df, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_derivatives_ddz", f, df, bc1, bcn, prec, stream)
+
+    def d2dx2(self, f, df, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_derivatives_d2dx2", f, df, bc1, bcn, prec, stream)
+
+    def d2dy2(self, f, df, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_derivatives_d2dy2", f, df, bc1, bcn, prec, stream)
+
+    def d2dz2(self, f, df, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_derivatives_d2dz2", f, df, bc1, bcn, prec, stream)
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_derivatives_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class filters:
+    """type(filters), src/filters/filters.F90 (serial init)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def init(self, nx, ny, nz, periodicx, periodicy, periodicz, methodx, methody, methodz):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        meth = [FIL_METHOD.get(m, 0) for m in (methodx, methody, methodz)]
+        return _check(lib().pdo_filters_init(ctypes.byref(self._h), nx, ny, nz,
+                                             _ints([periodicx, periodicy, periodicz]), _ints(meth)))
+
+    def _method(self, d):
+        m = ctypes.c_int()
+        _check(lib().pdo_filters_method(self._h, d, ctypes.byref(m)))
+        return {v: k for k, v in FIL_METHOD.items()}[m.value]
+
+    def getMethodx(self):
+        return self._method(1)
+
+    def getMethody(self):
+        return self._method(2)
+
+    def getMethodz(self):
+        return self._method(3)
+
+    def _ap(self, name, f, ff, bc1, bcn, prec, stream):
+        _field_check(self, f, ff)
+        return _check(getattr(lib(), name)(self._h, _prec(f, prec), _ptr(f), _ptr(ff), bc1, bcn, _stream(stream)))
+
+    def filterx(self, f, ff, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_filters_filterx", f, ff, bc1, bcn, prec, stream)
+
+    def filtery(self, f, ff, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_filters_filtery", f, ff, bc1, bcn, prec, stream)
+
+    def filterz(self, f, ff, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        return self._ap("pdo_filters_filterz", f, ff, bc1, bcn, prec, stream)
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_filters_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def _bcs(x_bc, y_bc, z_bc):
+    return _ints(list(x_bc) + list(y_bc) + list(z_bc))
+
+
+def gradient(der, f, dfdx, dfdy, dfdz, x_bc=(0, 0), y_bc=(0, 0), z_bc=(0, 0), prec="f32_refine", stream=None):
+    """operators%gradient (src/utilities/operators.F90:18-53)."""
+    _field_check(der, f, dfdx, dfdy, dfdz)
+    return _check(lib().pdo_operators_gradient(der._h, _prec(f, prec), _ptr(f), _ptr(dfdx), _ptr(dfdy), _ptr(dfdz),
+                                               _bcs(x_bc, y_bc, z_bc), _stream(stream)))
+
+
+def curl(der, u, v, w, curlu, x_bc=(0, 0), y_bc=(0, 0), z_bc=(0, 0), prec="f32_refine", stream=None):
+    """operators%curl (operators.F90:55-116); curlu holds 3 consecutive fields."""
+    _field_check(der, u, v, w)
+    if curlu.numel() < 3 * der.nx * der.ny * der.nz:
+        raise ValueError("curl output must hold three fields")
+    return _check(lib().pdo_operators_curl(der._h, _prec(u, prec), _ptr(u), _ptr(v), _ptr(w), _ptr(curlu),
+                                           _bcs(x_bc, y_bc, z_bc), _stream(stream)))
+
+
+def divergence(der, u, v, w, div, x_bc=(0, 0), y_bc=(0, 0), z_bc=(0, 0), prec="f32_refine", stream=None):
+    """operators%divergence (operators.F90:118-156)."""
+    _field_check(der, u, v, w, div)
+    return _check(lib().pdo_operators_divergence(der._h, _prec(u, prec), _ptr(u), _ptr(v), _ptr(w), _ptr(div),
+                                                 _bcs(x_bc, y_bc, z_bc), _stream(stream)))
+
+
+def filter3D(fil, arr, numtimes=1, x_bc=(0, 0), y_bc=(0, 0), z_bc=(0, 0), prec="f32_refine", stream=None):
+    """operators%filter3D (operators.F90:158-223): in place."""
+    _field_check(fil, arr)
+    return _check(lib().pdo_operators_filter3d(fil._h, _prec(arr, prec), _ptr(arr), numtimes,
+                                               _bcs(x_bc, y_bc, z_bc), _stream(stream)))
 
 
 class FullRHS:

@@ -48,6 +48,33 @@ SIGNATURES = {
     "pdo_cf90_filter2": (_APPLY_BC, _c_int),
     "pdo_cf90_filter3": (_APPLY_BC, _c_int),
     "pdo_cf90_filter3_cmplx": (_APPLY_BC, _c_int),
+    "pdo_gaussian_init": ([_pvp, _c_int, _c_int], _c_int),
+    "pdo_lstsq_init": ([_pvp, _c_int, _c_int], _c_int),
+    "pdo_xfilter_filter1": (_APPLY_BC, _c_int),
+    "pdo_xfilter_filter2": (_APPLY_BC, _c_int),
+    "pdo_xfilter_filter3": (_APPLY_BC, _c_int),
+    "pdo_xfilter_filter3_cmplx": (_APPLY_BC, _c_int),
+    "pdo_xfilter_destroy": ([_vp], _c_int),
+    "pdo_xfilter_dense": ([_vp, _c_int, _c_int, _pdbl], _c_int),
+    "pdo_derivatives_init": ([_pvp, _c_int, _c_int, _c_int, _c_double, _c_double, _c_double, _pint, _pint], _c_int),
+    "pdo_derivatives_method": ([_vp, _c_int, _pint], _c_int),
+    "pdo_derivatives_ddx": ([_vp, _c_int, _vp, _vp, _c_int, _c_int, _vp], _c_int),
+    "pdo_derivatives_ddy": ([_vp, _c_int, _vp, _vp, _c_int, _c_int, _vp], _c_int),
+    "pdo_derivatives_ddz": ([_vp, _c_int, _vp, _vp, _c_int, _c_int, _vp], _c_int),
+    "pdo_derivatives_d2dx2": ([_vp, _c_int, _vp, _vp, _c_int, _c_int, _vp], _c_int),
+    "pdo_derivatives_d2dy2": ([_vp, _c_int, _vp, _vp, _c_int, _c_int, _vp], _c_int),
+    "pdo_derivatives_d2dz2": ([_vp, _c_int, _vp, _vp, _c_int, _c_int, _vp], _c_int),
+    "pdo_derivatives_destroy": ([_vp], _c_int),
+    "pdo_filters_init": ([_pvp, _c_int, _c_int, _c_int, _pint, _pint], _c_int),
+    "pdo_filters_method": ([_vp, _c_int, _pint], _c_int),
+    "pdo_filters_filterx": ([_vp, _c_int, _vp, _vp, _c_int, _c_int, _vp], _c_int),
+    "pdo_filters_filtery": ([_vp, _c_int, _vp, _vp, _c_int, _c_int, _vp], _c_int),
+    "pdo_filters_filterz": ([_vp, _c_int, _vp, _vp, _c_int, _c_int, _vp], _c_int),
+    "pdo_filters_destroy": ([_vp], _c_int),
+    "pdo_operators_gradient": ([_vp, _c_int, _vp, _vp, _vp, _vp, _pint, _vp], _c_int),
+    "pdo_operators_curl": ([_vp, _c_int, _vp, _vp, _vp, _vp, _pint, _vp], _c_int),
+    "pdo_operators_divergence": ([_vp, _c_int, _vp, _vp, _vp, _vp, _pint, _vp], _c_int),
+    "pdo_operators_filter3d": ([_vp, _c_int, _vp, _c_int, _pint, _vp], _c_int),
     "pdo_cd06stagg_init_periodic": ([_pvp, _c_int, _c_double], _c_int),
     "pdo_cd06stagg_ddz_E2C": (_STAGG, _c_int),
     "pdo_cd06stagg_ddz_C2E": (_STAGG, _c_int),

@@ -246,12 +246,22 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
         // the factor is formed in the arithmetic of the transform (an FP64
         // division per point costs as much as the transforms in FP32)
         const int kx = (int)a, ky = (int)blockIdx.y;
-        const TC fxy = act ? (TC)(pm.kind == 2 ? pm.ax[kx] * pm.ay[ky] : pm.ax[kx] + pm.ay[ky]) : TC(0);
+        const bool line_only = pm.kind >= 3;  // factors of the line index alone
+        const TC fxy = (act && !line_only) ? (TC)(pm.kind == 2 ? pm.ax[kx] * pm.ay[ky] : pm.ax[kx] + pm.ay[ky])
+                                           : TC(0);
         const TC scale = (TC)pm.scale;
         auto sts = [&](int i, C2<TC> v) {
             TC fac;
             const TC az = (TC)pm.az[i];
-            if (pm.kind == 2) {
+            if (pm.kind == 3) {
+                // i az: (x + i y) i a = -a y + i a x
+                const TC s = az * scale;
+                buf(i) = C2<TC>{-s * v.y, s * v.x};
+                return;
+            }
+            if (pm.kind == 4) {
+                fac = az;
+            } else if (pm.kind == 2) {
                 fac = fxy * az;
             } else {
                 const TC ks = fxy + az;
@@ -567,6 +577,80 @@ hipError_t run_zmul(const PoissonPlanDev &pl, int mode, const void *in, void *ou
     case 0: return run_zmul_t<double, double>(pl, (const double *)in, (double *)out, m, st);
     case 1: return run_zmul_t<float, float>(pl, (const float *)in, (float *)out, m, st);
     case 2: return run_zmul_t<float, double>(pl, (const float *)in, (float *)out, m, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+// x lines of the half spectrum (pitch nxp complex): v *= i kv[kx] (imag) or kv[kx]
+template <typename TS>
+__global__ void __launch_bounds__(256) k_xline_mul(TS *spec, int nxp, int nxh, long nlines, const double *kv, int imag)
+{
+    const long total = (long)nxh * nlines;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const long line = e / nxh;
+        const int kx = (int)(e - line * nxh);
+        TS *p = spec + 2 * (line * nxp + kx);
+        const double k = kv[kx];
+        const double re = (double)p[0], im = (double)p[1];
+        if (imag) {
+            p[0] = (TS)(-k * im);
+            p[1] = (TS)(k * re);
+        } else {
+            p[0] = (TS)(k * re);
+            p[1] = (TS)(k * im);
+        }
+    }
+}
+
+template <typename TS, typename TC>
+static hipError_t run_fourier_deriv_t(const PoissonPlanDev &pl, int dir, int imag, const TS *in, TS *out, TS *work,
+                                      long na, long nb, const double *kv, hipStream_t st)
+{
+    const Tw<TC> tw(pl);
+    if (dir == 1) {
+        // r2c lines, multiply, c2r: the c2r returns L times the inverse transform
+        const int nx = pl.nx, L = nx / 2, nxh = L + 1, nxp = pl.nxp;
+        const long nlines = na * nb;
+        x_fft<TS, TC>(L, false, in, work, nlines, tw.x, tw.xn, (TC)1, nxp, 0, st);
+        long blocks = ((long)nxh * nlines + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL((k_xline_mul<TS>), dim3((unsigned)blocks), dim3(256), 0, st, work, nxp, nxh, nlines, kv,
+                           imag);
+        x_fft<TS, TC>(L, true, work, out, nlines, tw.x, tw.xn, (TC)(1.0 / L), nxp, 0, st);
+        return hipGetLastError();
+    }
+    // y / z: consecutive pairs along the contiguous index read as one complex
+    // value (the operator is real and maps real lines to real lines, so the
+    // real and imaginary parts come back as the two derivatives); one fused
+    // forward -> multiply -> inverse pass along the direction
+    SpecMul pm{};
+    pm.az = kv;
+    pm.kind = imag ? 3 : 4;
+    if (dir == 2) {
+        const int n = pl.ny;
+        pm.scale = 1.0 / n;
+        strided_fft<TS, TC, 2>(n, in, out, na / 2, (long)(na / 2) * n, na / 2, tw.y, pm, (int)nb, st);
+    } else {
+        const int n = pl.nz;
+        pm.scale = 1.0 / n;
+        strided_fft<TS, TC, 2>(n, in, out, na * nb / 2, 0, na * nb / 2, tw.z, pm, 1, st);
+    }
+    return hipGetLastError();
+}
+
+hipError_t run_fourier_deriv(const PoissonPlanDev &pl, int mode, int dir, int imag, const void *in, void *out,
+                             void *work, long na, long nb, const double *kv, hipStream_t st)
+{
+    switch (mode) {
+    case 0:
+        return run_fourier_deriv_t<double, double>(pl, dir, imag, (const double *)in, (double *)out, (double *)work,
+                                                   na, nb, kv, st);
+    case 1:
+        return run_fourier_deriv_t<float, float>(pl, dir, imag, (const float *)in, (float *)out, (float *)work, na,
+                                                 nb, kv, st);
+    case 2:
+        return run_fourier_deriv_t<float, double>(pl, dir, imag, (const float *)in, (float *)out, (float *)work, na,
+                                                  nb, kv, st);
     }
     return hipErrorInvalidValue;
 }

@@ -10,6 +10,8 @@ namespace pdo {
 //   kind 0: -1/(ax[kx]+ay[ky]+az[kz]+1e-20), zero at (0,0,0)    (PoissonPeriodic)
 //   kind 1: -1/(ax+ay+az), zero where ax+ay+az <= 1e-14          (PadePoisson)
 //   kind 2: ax[kx]*ay[ky]*az[kz]                                   (dealias mask)
+//   kind 3: i*az[kz]       (Fourier first derivative; ax, ay unused)
+//   kind 4: az[kz]         (Fourier second derivative; ax, ay unused)
 // then times `scale`.
 struct SpecMul {
     const double *ax, *ay, *az;
@@ -42,5 +44,13 @@ hipError_t run_ifft2(const PoissonPlanDev &pl, int mode, const void *in, void *w
 // z transform, multiply by m, inverse z transform of an unpadded
 // (nx/2+1, ny, nz) spectrum (in may equal out).
 hipError_t run_zmul(const PoissonPlanDev &pl, int mode, const void *in, void *out, const SpecMul &m, hipStream_t st);
+
+// Fourier-collocation derivative along dir of real f (n along dir; x:
+// (nx, na, nb), y: (na, ny, nb), z: (na, nb, nz)): the spectrum is multiplied
+// by i kv (imag != 0) or kv, kv holding nx/2+1 (x) or n (y, z) values with
+// the n/2 mode zero.  x needs work of nxp * na * nb complex values; y and z
+// need na (y) or na*nb (z) even.
+hipError_t run_fourier_deriv(const PoissonPlanDev &pl, int mode, int dir, int imag, const void *in, void *out,
+                             void *work, long na, long nb, const double *kv, hipStream_t st);
 
 }  // namespace pdo
