@@ -36,7 +36,7 @@ def _ref_fil(method, f, d, periodic, bc1, bcn):
 
 CASES = [
     # (shape, periodic xyz, methods xyz)
-    ((32, 16, 24), (False, True, True), ("cd10", "cd06", "four")),
+    ((32, 16, 32), (False, True, True), ("cd10", "cd06", "four")),
     ((16, 32, 16), (True, True, False), ("four", "four", "cd10")),
     ((24, 20, 32), (True, False, True), ("cd06", "cd10", "four")),
 ]
@@ -132,7 +132,7 @@ def test_filters_dispatch(cuda, case):
 
 def test_operators(cuda):
     import torch
-    shape = (32, 16, 24)
+    shape = (32, 16, 32)
     per = (False, True, True)
     meth = ("cd10", "cd06", "four")
     dxs = [L / n for n in shape]

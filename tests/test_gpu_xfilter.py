@@ -11,7 +11,7 @@ from fields import from_dev, random_field, rel_l2, to_dev
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(16, 8, 6), (8, 9, 5), (33, 12, 10), (64, 64, 64)]
+SHAPES = [(16, 8, 10), (8, 9, 12), (33, 12, 10), (64, 64, 64)]
 
 
 def _obj(kind, n, periodic):
