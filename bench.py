@@ -289,8 +289,9 @@ def committed_traffic(kernel):
 
 
 def cpu_baseline():
-    """The oracle's FP64 full RHS (OpenMP C) on the host: one evaluation at
-    256^3, scaled by the point count to a 512^3 evaluation rate."""
+    """The oracle's FP64 full RHS (OpenMP C) on the host: evaluations at
+    256^3 repeated for at least 10 s (at most 8), the mean scaled by the
+    point count to a 512^3 evaluation rate."""
     import numpy as np
 
     import oracle as O
@@ -298,13 +299,18 @@ def cpu_baseline():
     n = 256
     rng = np.random.default_rng(3)
     u, v, w = (np.asfortranarray(rng.standard_normal((n, n, n))) for _ in range(3))
-    t0 = time.perf_counter()
-    O.full_rhs(u, v, w)
-    t = time.perf_counter() - t0
-    return {"value": round(1.0 / (t * 8.0), 5), "unit": "RHS evals/s (512^3-equivalent)",
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        O.full_rhs(u, v, w)
+        reps += 1
+        t = time.perf_counter() - t0
+        if t >= 10.0 or reps >= 8:
+            break
+    per = t / reps
+    return {"value": round(1.0 / (per * 8.0), 5), "unit": "RHS evals/s (512^3-equivalent)",
             "cores": O.num_threads(), "kind": "port",
-            "sample": "one FP64 full-RHS evaluation of the C oracle at 256^3 (%.1f s), rate scaled by 1/8 to "
-                      "512^3 points" % t}
+            "sample": "%d FP64 full-RHS evaluations of the C oracle at 256^3 (%.1f s in all), rate scaled by "
+                      "1/8 to 512^3 points" % (reps, t)}
 
 
 if __name__ == "__main__":
