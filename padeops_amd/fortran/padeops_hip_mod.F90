@@ -279,6 +279,155 @@ module padeops_hip_mod
             type(c_ptr), value :: h, f, df, stream
             integer(c_int), value :: prec, n1, n2, is_complex
         end function
+        ! explicit filters, derivatives / filters dispatch, operators module
+        integer(c_int) function pdo_gaussian_init(h, n, periodic) &
+                bind(C, name="pdo_gaussian_init")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr) :: h
+            integer(c_int), value :: n, periodic
+        end function
+        integer(c_int) function pdo_lstsq_init(h, n, periodic) &
+                bind(C, name="pdo_lstsq_init")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr) :: h
+            integer(c_int), value :: n, periodic
+        end function
+        integer(c_int) function pdo_xfilter_filter1(h, prec, f, fil, na, nb, bc1, bcn, stream) &
+                bind(C, name="pdo_xfilter_filter1")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h, f, fil, stream
+            integer(c_int), value :: prec, na, nb, bc1, bcn
+        end function
+        integer(c_int) function pdo_xfilter_filter2(h, prec, f, fil, na, nb, bc1, bcn, stream) &
+                bind(C, name="pdo_xfilter_filter2")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h, f, fil, stream
+            integer(c_int), value :: prec, na, nb, bc1, bcn
+        end function
+        integer(c_int) function pdo_xfilter_filter3(h, prec, f, fil, na, nb, bc1, bcn, stream) &
+                bind(C, name="pdo_xfilter_filter3")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h, f, fil, stream
+            integer(c_int), value :: prec, na, nb, bc1, bcn
+        end function
+        integer(c_int) function pdo_xfilter_filter3_cmplx(h, prec, f, fil, na, nb, bc1, bcn, stream) &
+                bind(C, name="pdo_xfilter_filter3_cmplx")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h, f, fil, stream
+            integer(c_int), value :: prec, na, nb, bc1, bcn
+        end function
+        integer(c_int) function pdo_xfilter_destroy(h) &
+                bind(C, name="pdo_xfilter_destroy")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function pdo_derivatives_init(h, nx, ny, nz, dx, dy, dz, periodic, method) &
+                bind(C, name="pdo_derivatives_init")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr) :: h
+            integer(c_int), value :: nx, ny, nz
+            real(c_double), value :: dx, dy, dz
+            integer(c_int), dimension(3) :: periodic, method
+        end function
+        integer(c_int) function pdo_derivatives_ddx(h, prec, f, df, bc1, bcn, stream) &
+                bind(C, name="pdo_derivatives_ddx")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h, f, df, stream
+            integer(c_int), value :: prec, bc1, bcn
+        end function
+        integer(c_int) function pdo_derivatives_ddy(h, prec, f, df, bc1, bcn, stream) &
+                bind(C, name="pdo_derivatives_ddy")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h, f, df, stream
+            integer(c_int), value :: prec, bc1, bcn
+        end function
+        integer(c_int) function pdo_derivatives_ddz(h, prec, f, df, bc1, bcn, stream) &
+                bind(C, name="pdo_derivatives_ddz")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h, f, df, stream
+            integer(c_int), value :: prec, bc1, bcn
+        end function
+        integer(c_int) function pdo_derivatives_d2dx2(h, prec, f, df, bc1, bcn, stream) &
+                bind(C, name="pdo_derivatives_d2dx2")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h, f, df, stream
+            integer(c_int), value :: prec, bc1, bcn
+        end function
+        integer(c_int) function pdo_derivatives_d2dy2(h, prec, f, df, bc1, bcn, stream) &
+                bind(C, name="pdo_derivatives_d2dy2")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h, f, df, stream
+            integer(c_int), value :: prec, bc1, bcn
+        end function
+        integer(c_int) function pdo_derivatives_d2dz2(h, prec, f, df, bc1, bcn, stream) &
+                bind(C, name="pdo_derivatives_d2dz2")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h, f, df, stream
+            integer(c_int), value :: prec, bc1, bcn
+        end function
+        integer(c_int) function pdo_derivatives_destroy(h) &
+                bind(C, name="pdo_derivatives_destroy")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function pdo_filters_init(h, nx, ny, nz, periodic, method) &
+                bind(C, name="pdo_filters_init")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr) :: h
+            integer(c_int), value :: nx, ny, nz
+            integer(c_int), dimension(3) :: periodic, method
+        end function
+        integer(c_int) function pdo_filters_filterx(h, prec, f, ff, bc1, bcn, stream) &
+                bind(C, name="pdo_filters_filterx")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h, f, ff, stream
+            integer(c_int), value :: prec, bc1, bcn
+        end function
+        integer(c_int) function pdo_filters_filtery(h, prec, f, ff, bc1, bcn, stream) &
+                bind(C, name="pdo_filters_filtery")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h, f, ff, stream
+            integer(c_int), value :: prec, bc1, bcn
+        end function
+        integer(c_int) function pdo_filters_filterz(h, prec, f, ff, bc1, bcn, stream) &
+                bind(C, name="pdo_filters_filterz")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h, f, ff, stream
+            integer(c_int), value :: prec, bc1, bcn
+        end function
+        integer(c_int) function pdo_filters_destroy(h) &
+                bind(C, name="pdo_filters_destroy")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function pdo_operators_gradient(der, prec, f, dfdx, dfdy, dfdz, bc, stream) &
+                bind(C, name="pdo_operators_gradient")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: der, f, dfdx, dfdy, dfdz, stream
+            integer(c_int), value :: prec
+            integer(c_int), dimension(6) :: bc
+        end function
+        integer(c_int) function pdo_operators_curl(der, prec, u, v, w, curl, bc, stream) &
+                bind(C, name="pdo_operators_curl")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: der, u, v, w, curl, stream
+            integer(c_int), value :: prec
+            integer(c_int), dimension(6) :: bc
+        end function
+        integer(c_int) function pdo_operators_divergence(der, prec, u, v, w, div, bc, stream) &
+                bind(C, name="pdo_operators_divergence")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: der, u, v, w, div, stream
+            integer(c_int), value :: prec
+            integer(c_int), dimension(6) :: bc
+        end function
+        integer(c_int) function pdo_operators_filter3d(fil, prec, arr, numtimes, bc, stream) &
+                bind(C, name="pdo_operators_filter3d")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: fil, arr, stream
+            integer(c_int), value :: prec, numtimes
+            integer(c_int), dimension(6) :: bc
+        end function
     end interface
 
 contains
