@@ -280,6 +280,57 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
     }
 }
 
+// LDS layout of an x line: one pad element after every 8, so the radix-8
+// Stockham stores (8 j + r and 64 m + k + 8 r across the lanes j of a line)
+// fall in distinct banks
+#ifndef PDO_FFT_X_PAD
+#define PDO_FFT_X_PAD 1
+#endif
+__host__ __device__ constexpr int xpad(int i) { return PDO_FFT_X_PAD ? i + (i >> 3) : i; }
+__host__ __device__ constexpr int xline_pitch(int L) { return xpad(L) + 1; }
+
+// Z_k = E_k + i O_k of the half-length inverse transform from the staged
+// modes X_k (in place, one thread per pair (k, L-k)):
+// E = (X_k + conj X_{L-k})/2, O = (X_k - conj X_{L-k}) conj(W^k)/2.  Only the
+// real parts of the kx = 0 and kx = nx/2 modes enter (a Hermitian c2r);
+// oddball != 0 drops kx = nx/2.
+template <typename TC, int L, class BUF>
+__device__ __forceinline__ void c2r_merge(BUF &buf, const C2<TC> *wN, int jb, int oddball)
+{
+    constexpr int L8 = L >> 3;
+    for (int k = jb; k <= L / 2; k += L8) {
+        const int kc = L - k;
+        C2<TC> xk = buf(k), xc = buf(kc);
+        if (k == 0) {
+            xk.y = 0;
+            xc.y = 0;
+            if (oddball) xc.x = 0;
+        }
+        auto z = [&](C2<TC> a, C2<TC> b, int kk) {
+            const C2<TC> ev = {(TC)0.5 * (a.x + b.x), (TC)0.5 * (a.y - b.y)};
+            const C2<TC> df = {(TC)0.5 * (a.x - b.x), (TC)0.5 * (a.y + b.y)};
+            const C2<TC> od = cmul(df, C2<TC>{wN[kk].x, -wN[kk].y});
+            return C2<TC>{ev.x - od.y, ev.y + od.x};
+        };
+        const C2<TC> zk = z(xk, xc, k);
+        const C2<TC> zc = z(xc, xk, kc);
+        buf(k) = zk;
+        if (k > 0 && kc != k) buf(kc) = zc;
+    }
+}
+
+// X_k = E_k + W^k O_k of the real transform from the half-length one in buf,
+// E = (Z_k + conj Z_{L-k})/2, O = (Z_k - conj Z_{L-k})/(2i)
+template <typename TC, int L, class BUF>
+__device__ __forceinline__ C2<TC> r2c_split(BUF &buf, const C2<TC> *wN, int k)
+{
+    const C2<TC> zk = buf(k == L ? 0 : k);
+    const C2<TC> zc = buf(k == 0 ? 0 : L - k);
+    const C2<TC> ev = {(TC)0.5 * (zk.x + zc.x), (TC)0.5 * (zk.y - zc.y)};
+    const C2<TC> od = {(TC)0.5 * (zk.y + zc.y), (TC)-0.5 * (zk.x - zc.x)};
+    return cadd(ev, cmul(wN[k], od));
+}
+
 // ---------------------------------------------------------------------------
 // pass 1: x real-to-complex.  nlines lines of nx reals -> nx/2+1 complex.
 // Tile: TXL lines; thread = (line, jb), jb fastest.
@@ -291,28 +342,21 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c(const TS *__restrict__ in, T
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
-    constexpr int nx = 2 * L, L8 = L >> 3, LDL = L + 1, nxh = L + 1;
+    constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch(L), nxh = L + 1;
     const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
     const long l = (long)blockIdx.x * TXL + line;
     const bool act = (line < TXL) && (l < nlines);
     using IO = CLoad<TS, TC>;
     const TS *src = in + (act ? l : 0) * (long)nx;
     auto ldg = [&](int i) { return IO::ld(src, i); };  // packed z_j = (x_2j, x_2j+1)
-    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + i]; };
+    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xpad(i)]; };
     auto sts = [&](int i, C2<TC> v) { buf(i) = v; };
     line_fft<TC, -1, false, L>(ldg, sts, buf, (const C2<TC> *)twL, jb, act);
     __syncthreads();
-    // split: X_k = E_k + W^k O_k, E = (Z_k + conj Z_{L-k})/2, O = (Z_k - conj Z_{L-k})/(2i)
     const C2<TC> *wN = (const C2<TC> *)twN;
     TS *dst = out + (act ? l : 0) * (long)nxp * 2;
     if (act) {
-        for (int k = jb; k < nxh; k += L8) {
-            const C2<TC> zk = buf(k == L ? 0 : k);
-            const C2<TC> zc = buf(k == 0 ? 0 : L - k);
-            const C2<TC> ev = {(TC)0.5 * (zk.x + zc.x), (TC)0.5 * (zk.y - zc.y)};
-            const C2<TC> od = {(TC)0.5 * (zk.y + zc.y), (TC)-0.5 * (zk.x - zc.x)};
-            IO::st(dst, k, cadd(ev, cmul(wN[k], od)));
-        }
+        for (int k = jb; k < nxh; k += L8) IO::st(dst, k, r2c_split<TC, L>(buf, wN, k));
     }
 }
 
@@ -326,13 +370,13 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r(const TS *__restrict__ in, T
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
-    constexpr int nx = 2 * L, L8 = L >> 3, LDL = L + 1;
+    constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch(L);
     const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
     const long l = (long)blockIdx.x * TXL + line;
     const bool act = (line < TXL) && (l < nlines);
     using IO = CLoad<TS, TC>;
     const TS *src = in + (act ? l : 0) * (long)nxp * 2;
-    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + i]; };
+    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xpad(i)]; };
     const C2<TC> *wN = (const C2<TC> *)twN;
     // stage the nx/2+1 modes of the line in LDS (coalesced, all loads of a
     // thread in flight together), then form Z_k = E_k + i O_k in place, one
@@ -349,27 +393,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r(const TS *__restrict__ in, T
         if (jb == 0) buf(L) = vl;
     }
     __syncthreads();
-    if (act) {
-        for (int k = jb; k <= L / 2; k += L8) {
-            const int kc = L - k;
-            C2<TC> xk = buf(k), xc = buf(kc);
-            if (k == 0) {
-                xk.y = 0;
-                xc.y = 0;
-                if (oddball) xc.x = 0;
-            }
-            auto z = [&](C2<TC> a, C2<TC> b, int kk) {
-                const C2<TC> ev = {(TC)0.5 * (a.x + b.x), (TC)0.5 * (a.y - b.y)};
-                const C2<TC> df = {(TC)0.5 * (a.x - b.x), (TC)0.5 * (a.y + b.y)};
-                const C2<TC> od = cmul(df, C2<TC>{wN[kk].x, -wN[kk].y});
-                return C2<TC>{ev.x - od.y, ev.y + od.x};
-            };
-            const C2<TC> zk = z(xk, xc, k);
-            const C2<TC> zc = z(xc, xk, kc);
-            buf(k) = zk;
-            if (k > 0 && kc != k) buf(kc) = zc;
-        }
-    }
+    if (act) c2r_merge<TC, L>(buf, wN, jb, oddball);
     __syncthreads();
     TS *dst = out + (act ? l : 0) * (long)nx;
     auto ldb = [&](int i) { return buf(i); };
@@ -379,6 +403,127 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r(const TS *__restrict__ in, T
         IO::st(dst, i, v);
     };
     line_fft<TC, +1, true, L>(ldb, stg, buf, (const C2<TC> *)twL, jb, act);
+}
+
+// The x transforms of the full-RHS Poisson solve with the x derivatives of
+// its divergence and of its projection taken in spectral space.  On a
+// periodic grid the cd06 first derivative is diagonal in Fourier space, with
+// eigenvalue i k'(kx) on mode kx (k' the modified wavenumber, zero at kx = 0
+// and kx = nx/2), so these passes form the same operators as a compact x
+// solve, without the extra pass over HBM.
+//
+// r2c: X = FFT_x(div) + i k'(kx) FFT_x(ut), line by line.
+template <typename TS, typename TC, int L>
+__global__ void __launch_bounds__(1024) k_fft_x_r2c_dx(const TS *__restrict__ div, const TS *__restrict__ ut,
+                                                      TS *__restrict__ out, long nlines, const TC *__restrict__ twL,
+                                                      const TC *__restrict__ twN, int TXL, int nxp,
+                                                      const double *__restrict__ kp)
+{
+    extern __shared__ unsigned char smem[];
+    C2<TC> *tile = (C2<TC> *)smem;
+    constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch(L), nxh = L + 1;
+    constexpr int NK = (nxh + L8 - 1) / L8;  // modes per thread
+    const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
+    const long l = (long)blockIdx.x * TXL + line;
+    const bool act = (line < TXL) && (l < nlines);
+    using IO = CLoad<TS, TC>;
+    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xpad(i)]; };
+    auto sts = [&](int i, C2<TC> v) { buf(i) = v; };
+    const C2<TC> *wN = (const C2<TC> *)twN;
+    // i k' FFT(ut), held in registers
+    C2<TC> du[NK];
+    {
+        const TS *src = ut + (act ? l : 0) * (long)nx;
+        auto ldg = [&](int i) { return IO::ld(src, i); };
+        line_fft<TC, -1, false, L>(ldg, sts, buf, (const C2<TC> *)twL, jb, act);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NK; ++q) {
+            const int k = jb + q * L8;
+            du[q] = C2<TC>{0, 0};
+            if (act && k < nxh) {
+                const C2<TC> x = r2c_split<TC, L>(buf, wN, k);
+                const TC kk = (TC)kp[k];
+                du[q] = C2<TC>{-kk * x.y, kk * x.x};
+            }
+        }
+        __syncthreads();  // every read of the buffer is done
+    }
+    const TS *src = div + (act ? l : 0) * (long)nx;
+    auto ldg = [&](int i) { return IO::ld(src, i); };
+    line_fft<TC, -1, false, L>(ldg, sts, buf, (const C2<TC> *)twL, jb, act);
+    __syncthreads();
+    TS *dst = out + (act ? l : 0) * (long)nxp * 2;
+    if (act) {
+#pragma unroll
+        for (int q = 0; q < NK; ++q) {
+            const int k = jb + q * L8;
+            if (k < nxh) IO::st(dst, k, cadd(r2c_split<TC, L>(buf, wN, k), du[q]));
+        }
+    }
+}
+
+// c2r: p = scale IFFT_x(P) and u = ut - scale IFFT_x(i k'(kx) P), line by line.
+template <typename TS, typename TC, int L>
+__global__ void __launch_bounds__(1024) k_fft_x_c2r_dx(const TS *__restrict__ in, TS *__restrict__ p,
+                                                      const TS *__restrict__ ut, TS *__restrict__ u, long nlines,
+                                                      const TC *__restrict__ twL, const TC *__restrict__ twN, int TXL,
+                                                      TC scale, int nxp, const double *__restrict__ kp)
+{
+    extern __shared__ unsigned char smem[];
+    C2<TC> *tile = (C2<TC> *)smem;
+    constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch(L);
+    const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
+    const long l = (long)blockIdx.x * TXL + line;
+    const bool act = (line < TXL) && (l < nlines);
+    using IO = CLoad<TS, TC>;
+    const TS *src = in + (act ? l : 0) * (long)nxp * 2;
+    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xpad(i)]; };
+    auto ldb = [&](int i) { return buf(i); };
+    const C2<TC> *wN = (const C2<TC> *)twN;
+    C2<TC> v[8];
+    C2<TC> vl = {};
+    if (act) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = IO::ld(src, jb + q * L8);
+        if (jb == 0) vl = IO::ld(src, L);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) buf(jb + q * L8) = v[q];
+        if (jb == 0) buf(L) = vl;
+    }
+    __syncthreads();
+    if (act) c2r_merge<TC, L>(buf, wN, jb, 0);
+    __syncthreads();
+    TS *pl = p + (act ? l : 0) * (long)nx;
+    auto stp = [&](int i, C2<TC> x) {
+        x.x *= scale;
+        x.y *= scale;
+        IO::st(pl, i, x);
+    };
+    line_fft<TC, +1, true, L>(ldb, stp, buf, (const C2<TC> *)twL, jb, act);
+    __syncthreads();  // every read of the buffer by the first transform is done
+    // i k' P, from the modes still in registers
+    if (act) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const TC kk = (TC)kp[jb + q * L8];
+            buf(jb + q * L8) = C2<TC>{-kk * v[q].y, kk * v[q].x};
+        }
+        if (jb == 0) {
+            const TC kk = (TC)kp[L];
+            buf(L) = C2<TC>{-kk * vl.y, kk * vl.x};
+        }
+    }
+    __syncthreads();
+    if (act) c2r_merge<TC, L>(buf, wN, jb, 0);
+    __syncthreads();
+    const TS *ul = ut + (act ? l : 0) * (long)nx;
+    TS *uo = u + (act ? l : 0) * (long)nx;
+    auto stu = [&](int i, C2<TC> x) {
+        const C2<TC> a = IO::ld(ul, i);
+        IO::st(uo, i, C2<TC>{a.x - scale * x.x, a.y - scale * x.y});
+    };
+    line_fft<TC, +1, true, L>(ldb, stu, buf, (const C2<TC> *)twL, jb, act);
 }
 
 // ---------------------------------------------------------------------------
@@ -443,13 +588,13 @@ static void launch_x_fft(bool inverse, const TS *in, TS *out, long nlines, const
     const size_t ce = 2 * sizeof(TC);
     int TXL = PDO_FFT_X_THREADS / (L / 8);
     if (TXL < 1) TXL = 1;
-    while (TXL > 1 && (size_t)TXL * (L + 1) * ce > 64 * 1024) TXL >>= 1;
+    while (TXL > 1 && (size_t)TXL * xline_pitch(L) * ce > 64 * 1024) TXL >>= 1;
     dim3 grid((unsigned)((nlines + TXL - 1) / TXL));
     if (!inverse)
-        hipLaunchKernelGGL((k_fft_x_r2c<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * (L + 1) * ce, st, in,
+        hipLaunchKernelGGL((k_fft_x_r2c<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch(L) * ce, st, in,
                            out, nlines, tw, twn, TXL, nxp);
     else
-        hipLaunchKernelGGL((k_fft_x_c2r<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * (L + 1) * ce, st, in,
+        hipLaunchKernelGGL((k_fft_x_c2r<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch(L) * ce, st, in,
                            out, nlines, tw, twn, TXL, scale, nxp, oddball);
 }
 
@@ -465,6 +610,39 @@ static void x_fft(int L, bool inverse, const TS *in, TS *out, long nlines, const
     case 128: launch_x_fft<TS, TC, 128>(inverse, in, out, nlines, tw, twn, scale, nxp, oddball, st); break;
     case 256: launch_x_fft<TS, TC, 256>(inverse, in, out, nlines, tw, twn, scale, nxp, oddball, st); break;
     case 512: launch_x_fft<TS, TC, 512>(inverse, in, out, nlines, tw, twn, scale, nxp, oddball, st); break;
+    }
+}
+
+// the x passes with the spectral x derivative (k_fft_x_r2c_dx / _c2r_dx)
+template <typename TS, typename TC, int L>
+static void launch_x_fft_dx(bool inverse, const TS *a, const TS *b, TS *o1, TS *o2, long nlines, const TC *tw,
+                            const TC *twn, TC scale, int nxp, const double *kp, hipStream_t st)
+{
+    const size_t ce = 2 * sizeof(TC);
+    int TXL = PDO_FFT_X_THREADS / (L / 8);
+    if (TXL < 1) TXL = 1;
+    while (TXL > 1 && (size_t)TXL * xline_pitch(L) * ce > 64 * 1024) TXL >>= 1;
+    dim3 grid((unsigned)((nlines + TXL - 1) / TXL));
+    if (!inverse)
+        hipLaunchKernelGGL((k_fft_x_r2c_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch(L) * ce, st, a,
+                           b, o1, nlines, tw, twn, TXL, nxp, kp);
+    else
+        hipLaunchKernelGGL((k_fft_x_c2r_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch(L) * ce, st, a,
+                           o1, b, o2, nlines, tw, twn, TXL, scale, nxp, kp);
+}
+
+template <typename TS, typename TC>
+static void x_fft_dx(int L, bool inverse, const TS *a, const TS *b, TS *o1, TS *o2, long nlines, const TC *tw,
+                     const TC *twn, TC scale, int nxp, const double *kp, hipStream_t st)
+{
+    switch (L) {
+    case 8: launch_x_fft_dx<TS, TC, 8>(inverse, a, b, o1, o2, nlines, tw, twn, scale, nxp, kp, st); break;
+    case 16: launch_x_fft_dx<TS, TC, 16>(inverse, a, b, o1, o2, nlines, tw, twn, scale, nxp, kp, st); break;
+    case 32: launch_x_fft_dx<TS, TC, 32>(inverse, a, b, o1, o2, nlines, tw, twn, scale, nxp, kp, st); break;
+    case 64: launch_x_fft_dx<TS, TC, 64>(inverse, a, b, o1, o2, nlines, tw, twn, scale, nxp, kp, st); break;
+    case 128: launch_x_fft_dx<TS, TC, 128>(inverse, a, b, o1, o2, nlines, tw, twn, scale, nxp, kp, st); break;
+    case 256: launch_x_fft_dx<TS, TC, 256>(inverse, a, b, o1, o2, nlines, tw, twn, scale, nxp, kp, st); break;
+    case 512: launch_x_fft_dx<TS, TC, 512>(inverse, a, b, o1, o2, nlines, tw, twn, scale, nxp, kp, st); break;
     }
 }
 
@@ -497,6 +675,41 @@ static hipError_t run_poisson_t(const PoissonPlanDev &pl, const TS *rhs, TS *f, 
     strided_fft<TS, TC, 1>(ny, spec, spec, (long)nxh, (long)nxp * ny, (long)nxp, tw.y, pm, nz, st);
     x_fft<TS, TC>(L, true, spec, f, nlines, tw.x, tw.xn, scale, nxp, 0, st);
     return hipGetLastError();
+}
+
+template <typename TS, typename TC>
+static hipError_t run_poisson_dx_t(const PoissonPlanDev &pl, const TS *div, const TS *ut, TS *p, TS *ux, TS *spec,
+                                   const double *kp, hipStream_t st)
+{
+    const int nx = pl.nx, ny = pl.ny, nz = pl.nz, nxh = nx / 2 + 1, L = nx / 2;
+    const Tw<TC> tw(pl);
+    const SpecMul pm{pl.kx2, pl.ky2, pl.kz2, pl.zero_policy, 1.0};
+    const long nlines = (long)ny * nz;
+    const TC scale = (TC)(1.0 / ((double)L * ny * nz));
+    const int nxp = pl.nxp;
+    x_fft_dx<TS, TC>(L, false, div, ut, spec, nullptr, nlines, tw.x, tw.xn, (TC)1, nxp, kp, st);
+    strided_fft<TS, TC, 0>(ny, spec, spec, (long)nxh, (long)nxp * ny, (long)nxp, tw.y, pm, nz, st);
+    strided_fft<TS, TC, 2>(nz, spec, spec, (long)nxh, (long)nxp, (long)nxp * ny, tw.z, pm, ny, st);
+    strided_fft<TS, TC, 1>(ny, spec, spec, (long)nxh, (long)nxp * ny, (long)nxp, tw.y, pm, nz, st);
+    x_fft_dx<TS, TC>(L, true, spec, ut, p, ux, nlines, tw.x, tw.xn, scale, nxp, kp, st);
+    return hipGetLastError();
+}
+
+hipError_t run_poisson_dx(const PoissonPlanDev &pl, int mode, const void *div, const void *ut, void *p, void *ux,
+                          void *spec, const double *kp, hipStream_t st)
+{
+    switch (mode) {
+    case 0:
+        return run_poisson_dx_t<double, double>(pl, (const double *)div, (const double *)ut, (double *)p,
+                                                (double *)ux, (double *)spec, kp, st);
+    case 1:
+        return run_poisson_dx_t<float, float>(pl, (const float *)div, (const float *)ut, (float *)p, (float *)ux,
+                                              (float *)spec, kp, st);
+    case 2:
+        return run_poisson_dx_t<float, double>(pl, (const float *)div, (const float *)ut, (float *)p, (float *)ux,
+                                               (float *)spec, kp, st);
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t run_poisson(const PoissonPlanDev &pl, int mode, const void *rhs, void *f, void *spec, hipStream_t st)

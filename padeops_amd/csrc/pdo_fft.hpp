@@ -34,6 +34,13 @@ struct PoissonPlanDev {
 // spec: work buffer of nxp*ny*nz complex values of the storage type.
 hipError_t run_poisson(const PoissonPlanDev &pl, int mode, const void *rhs, void *f, void *spec, hipStream_t st);
 
+// The Poisson solve of the full RHS with its x terms in spectral space:
+// p = Poisson(div + D_x ut), ux = ut - D_x p, where D_x is the operator with
+// eigenvalue i kp[kx] on x mode kx (kp: nx/2+1 values; for the cd06 first
+// derivative, its modified wavenumber, zero at kx = 0 and nx/2).
+hipError_t run_poisson_dx(const PoissonPlanDev &pl, int mode, const void *div, const void *ut, void *p, void *ux,
+                          void *spec, const double *kp, hipStream_t st);
+
 // 2D (x r2c, y c2c) forward transform of each of `nplanes` (nx, ny) planes into
 // an unpadded (nx/2+1, ny, nplanes) spectrum; unnormalized.
 hipError_t run_fft2(const PoissonPlanDev &pl, int mode, const void *in, void *out, int nplanes, hipStream_t st);

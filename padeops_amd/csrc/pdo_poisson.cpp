@@ -155,7 +155,30 @@ int PoissonObject::init(int nx_, int ny_, int nz_, double dx, double dy, double 
     allocs.push_back(p); dev.ky2 = p;
     if ((st = upload_vec(k2(nz, dz, mz, nz), &p))) return st;
     allocs.push_back(p); dev.kz2 = p;
+    {
+        std::vector<double> k = wavenums(nx, dx), kp(nxh);
+        for (int i = 0; i < nxh; i++) kp[i] = (i == 0 || i == nx / 2) ? 0.0 : modified_k(mx, k[i], dx);
+        if ((st = upload_vec(kp, &p))) return st;
+        allocs.push_back(p); kpx = p;
+    }
     return fft_tables(dev, nx, ny, nz, allocs);
+}
+
+int PoissonObject::solve_dx(int prec, const void *div, const void *ut, void *p, void *ux, hipStream_t st)
+{
+    if (!div || !ut || !p || !ux) { set_error("null field"); return PDO_ERR_ARG; }
+    const int mode = fft_mode(prec);
+    if (mode < 0) { set_error("unknown precision mode"); return PDO_ERR_ARG; }
+    const size_t need = (size_t)dev.nxp * ny * nz * 2 * (mode == 0 ? 8 : 4);
+    if (spec_bytes < need) {
+        if (spec) (void)hipFree(spec);
+        spec = nullptr; spec_bytes = 0;
+        hipError_t e = hipMalloc(&spec, need);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(spectrum)");
+        spec_bytes = need;
+    }
+    hipError_t e = run_poisson_dx(dev, mode, div, ut, p, ux, spec, kpx, st);
+    return e == hipSuccess ? 0 : hip_fail(e, "poisson kernels");
 }
 
 int PoissonObject::solve(int prec, const void *rhs, void *f, hipStream_t st)

@@ -25,8 +25,12 @@ struct PoissonObject {
     std::vector<void *> allocs;
     void *spec = nullptr;
     size_t spec_bytes = 0;
+    double *kpx = nullptr;  // x modified wavenumber k'(kx), zero at kx = 0 and nx/2 (device, nx/2+1)
     int init(int nx, int ny, int nz, double dx, double dy, double dz, int mx, int my, int mz, int zero_policy);
     int solve(int prec, const void *rhs, void *f, hipStream_t st);
+    // p = Poisson(div + D_x ut) and ux = ut - D_x p with D_x the x derivative
+    // whose modified wavenumber the solver uses (p may alias div)
+    int solve_dx(int prec, const void *div, const void *ut, void *p, void *ux, hipStream_t st);
     ~PoissonObject();
 };
 

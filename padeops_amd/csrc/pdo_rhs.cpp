@@ -17,6 +17,11 @@
 #ifndef PDO_RHS_SIDE_STREAM
 #define PDO_RHS_SIDE_STREAM 1
 #endif
+// form the x terms of the divergence and of the projection inside the
+// Poisson solve's x transforms (1) or as compact x passes (0)
+#ifndef PDO_RHS_SPECTRAL_X
+#define PDO_RHS_SPECTRAL_X 1
+#endif
 
 namespace pdo {
 
@@ -117,13 +122,26 @@ int pdo_rhs_eval(pdo_rhs h, const void *u, const void *v, const void *w, void *g
         if ((r = op(&h->fil, KIND_FILTER, 3, h->tmp, h->ut[i], 0, nullptr))) return r;
     }
     // divergence of the filtered velocity, accumulated in place
-    if ((r = op(&h->d1, KIND_D1, 1, h->ut[0], h->div, 0, nullptr))) return r;
-    if ((r = op(&h->d1, KIND_D1, 2, h->ut[1], h->div, 1, h->div))) return r;
-    if ((r = op(&h->d1, KIND_D1, 3, h->ut[2], h->div, 1, h->div))) return r;
-    // pressure (in place) and projection u = ut - grad p
-    if ((r = h->poisson.solve(h->prec, h->div, h->div, st))) return r;
-    for (int i = 0; i < 3; i++)
-        if ((r = op(&h->d1, KIND_D1, i + 1, h->div, uoi[i], 2, h->ut[i]))) return r;
+    if (PDO_RHS_SPECTRAL_X) {
+        // the z and y terms as compact passes; the x term, and the x
+        // derivative of the pressure, are taken by the Poisson solve's x
+        // transforms with the cd06 eigenvalues i k'(kx) (on the periodic
+        // grid the same operator as the compact x solve)
+        if ((r = op(&h->d1, KIND_D1, 3, h->ut[2], h->div, 0, nullptr))) return r;
+        if ((r = op(&h->d1, KIND_D1, 2, h->ut[1], h->div, 1, h->div))) return r;
+        // p = Poisson(div + d ut/dx) (in place) and u = ut - dp/dx
+        if ((r = h->poisson.solve_dx(h->prec, h->div, h->ut[0], h->div, uoi[0], st))) return r;
+        for (int i = 1; i < 3; i++)
+            if ((r = op(&h->d1, KIND_D1, i + 1, h->div, uoi[i], 2, h->ut[i]))) return r;
+    } else {
+        if ((r = op(&h->d1, KIND_D1, 1, h->ut[0], h->div, 0, nullptr))) return r;
+        if ((r = op(&h->d1, KIND_D1, 2, h->ut[1], h->div, 1, h->div))) return r;
+        if ((r = op(&h->d1, KIND_D1, 3, h->ut[2], h->div, 1, h->div))) return r;
+        // pressure (in place) and projection u = ut - grad p
+        if ((r = h->poisson.solve(h->prec, h->div, h->div, st))) return r;
+        for (int i = 0; i < 3; i++)
+            if ((r = op(&h->d1, KIND_D1, i + 1, h->div, uoi[i], 2, h->ut[i]))) return r;
+    }
     // everything after this call on the caller's stream also follows the gradients
     if ((e = hipStreamWaitEvent(st, h->ev_join, 0)) != hipSuccess) return hip_fail(e, "stream join");
     return 0;
