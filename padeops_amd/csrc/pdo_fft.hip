@@ -96,7 +96,7 @@ template <typename T, int R, int SIGN> struct Stage {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 C2<T> x = ld(j + r * Q);
-                if (r > 0) {
+                if (r > 0 && Ns > 1) {  // the first stage's twiddles are all 1
                     C2<T> w = tw[(k * r * step) & (L - 1)];
                     if (SIGN > 0) w.y = -w.y;
                     x = cmul(x, w);
