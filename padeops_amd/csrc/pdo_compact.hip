@@ -257,20 +257,26 @@ __device__ __forceinline__ void reduce_seg(const typename Mode<MODE>::TC *red, c
         const int D = (sizeof(TC) == 8) ? A.D64 : A.D32;
         const int nd = (2 * D + 1 < P) ? 2 * D + 1 : P;
         const int d0 = (2 * D + 1 < P) ? P - D : 0;
+        // segment t = s + dd (mod P) advances with dd: a running index with
+        // one wrap per pass, the 2K values of a segment at immediate offsets
+        int t = s + d0;
+        t -= (t >= P) ? P : 0;
         for (int id = 0; id < nd; ++id) {
             int dd = d0 + id;
             dd -= (dd >= P) ? P : 0;
-            int t = s + dd;
-            t -= (t >= P) ? P : 0;
+            const TC *zp = red + t * 2 * K * NPW + pl;
+            TC zv[2 * K];
+#pragma unroll
+            for (int r = 0; r < 2 * K; ++r) zv[r] = zp[r * NPW];
 #pragma unroll
             for (int r = 0; r < 2 * K; ++r) {
-                const TC zv = red[(t * 2 * K + r) * NPW + pl];
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
-                    yl[q] = fma(G[q * NR + dd * 2 * K + r], zv, yl[q]);
-                    yr[q] = fma(G[(K + q) * NR + dd * 2 * K + r], zv, yr[q]);
+                    yl[q] = fma(G[q * NR + dd * 2 * K + r], zv[r], yl[q]);
+                    yr[q] = fma(G[(K + q) * NR + dd * 2 * K + r], zv[r], yr[q]);
                 }
             }
+            t = (t == P - 1) ? 0 : t + 1;
         }
     } else {
         const TC *G = tab<TC>(A) + A.o_G + (size_t)s * 2 * K * NR;
@@ -558,32 +564,43 @@ __device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename 
     for (int j = 0; j < MM; ++j) d[j] = (TC)rhs_row<FAM, TR>(win, j, cr, A, ty, m);
 
     const int NR = 2 * K * P;
+    // exchange buffers hold value r of lane (p, s) at r * NT + p * P + s: the
+    // lanes of a half-wave touch consecutive words (conflict-free)
+    const int NT = TP * P;
     auto exchange = [&](TC *rb, TC (&v)[MM], TC (&yl)[2], TC (&yr)[2]) {
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            rb[(p * P + s) * 2 * K + q] = v[q];
-            rb[(p * P + s) * 2 * K + K + q] = v[m - K + q];
+            rb[q * NT + p * P + s] = v[q];
+            rb[(K + q) * NT + p * P + s] = v[m - K + q];
         }
         __syncthreads();
         yl[0] = yl[1] = yr[0] = yr[1] = TC(0);
         const int D = (sizeof(TC) == 8) ? A.D64 : A.D32;
         const int nd = (2 * D + 1 < P) ? 2 * D + 1 : P;
         const int d0 = (2 * D + 1 < P) ? P - D : 0;
+        // periodic systems are block circulant: G_s[.][t] = G_0[.][t - s].
+        // The segment t = s + dd (mod P) advances with dd, so the lane keeps
+        // a running LDS pointer (one wrap per pass) and the four values of a
+        // segment are immediate offsets from it; dd is uniform.
+        const TC *rp = rb + p * P;
+        int t = s + d0;
+        t -= (t >= P) ? P : 0;
         for (int id = 0; id < nd; ++id) {
             int dd = d0 + id;
             dd -= (dd >= P) ? P : 0;
-            // periodic systems are block circulant: G_s[.][t] = G_0[.][t - s]
-            const int t = PER ? (s + dd < P ? s + dd : s + dd - P) : dd;
-            const int col = PER ? dd : t;
+            const TC *zp = rp + t;
+            TC zv[2 * K];
+#pragma unroll
+            for (int r = 0; r < 2 * K; ++r) zv[r] = zp[r * NT];
 #pragma unroll
             for (int r = 0; r < 2 * K; ++r) {
-                const TC zv = rb[(p * P + t) * 2 * K + r];
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
-                    yl[q] = fma(G[q * NR + col * 2 * K + r], zv, yl[q]);
-                    yr[q] = fma(G[(K + q) * NR + col * 2 * K + r], zv, yr[q]);
+                    yl[q] = fma(G[q * NR + dd * 2 * K + r], zv[r], yl[q]);
+                    yr[q] = fma(G[(K + q) * NR + dd * 2 * K + r], zv[r], yr[q]);
                 }
             }
+            t = (t == P - 1) ? 0 : t + 1;
         }
     };
 
@@ -598,8 +615,8 @@ __device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename 
         (void)ty;
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            rb3[(p * P + s) * 2 * K + q] = d[q];
-            rb3[(p * P + s) * 2 * K + K + q] = d[m - K + q];
+            rb3[q * NT + p * P + s] = d[q];
+            rb3[(K + q) * NT + p * P + s] = d[m - K + q];
         }
         __syncthreads();
         {
@@ -607,8 +624,8 @@ __device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename 
             const int nx = s < P - 1 ? s + 1 : 0;
 #pragma unroll
             for (int q = 0; q < K; ++q) {
-                yl[q] = rb3[(p * P + pv) * 2 * K + K + q];
-                yr[q] = rb3[(p * P + nx) * 2 * K + q];
+                yl[q] = rb3[(K + q) * NT + p * P + pv];
+                yr[q] = rb3[q * NT + p * P + nx];
             }
         }
         TC e[MM];
@@ -896,7 +913,9 @@ static hipError_t launch_x_t(const ApplyArgs &A, hipStream_t st)
     constexpr int K = Fam<FAM>::K;
     const int P = A.P;
     int TP = 256 / P;
-    const int LD = A.n_in + (A.n_in >> 5) + 1;
+    // pencil pitch n + n/32 = 33 P: the P segments of consecutive pencils in
+    // a half-wave start 33 s + 33 P p words in, all in distinct banks
+    const int LD = A.n_in + (A.n_in >> 5);
     auto lds_of = [&](int tp) {
         return (size_t)tp * LD * sizeof(TS) +
                (size_t)tp * P * 2 * K * sizeof(TC) * (Mode<MODE>::REFINE ? PDO_XDIR_RED_SLOTS : 1);

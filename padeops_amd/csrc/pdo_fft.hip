@@ -23,7 +23,9 @@
 
 namespace pdo {
 
-template <typename T> struct C2 { T x, y; };
+// complex value; aligned to its size so that LDS and HBM accesses are single
+// 8-byte (FP32) or 16-byte (FP64) operations rather than pairs of words
+template <typename T> struct alignas(2 * sizeof(T)) C2 { T x, y; };
 
 template <typename T> __device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b)
 {
@@ -211,6 +213,11 @@ template <> struct CLoad<float, double> : CLoadF<double> {};
 __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// LDS rows of the strided transforms swizzled (1) or plain (0); tuning constant
+#ifndef PDO_FFT_STRIDED_SWZ
+#define PDO_FFT_STRIDED_SWZ 1
+#endif
+
 // ---------------------------------------------------------------------------
 // passes 2-4: complex FFT of strided lines.  Line (a, b) starts at a + b*sb
 // (complex units), element stride se; a in [0, na) is the contiguous index.
@@ -235,7 +242,13 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
     using IO = CLoad<TS, TC>;
     auto ldg = [&](int i) { return IO::ld(lin, i * se); };
     auto stg = [&](int i, C2<TC> v) { IO::st(lout, i * se, v); };
-    auto buf = [&](int i) -> C2<TC> & { return tile[i * TX + ln]; };
+    // element i of the line at row i ^ ((i >> 3) & 1): the first Stockham
+    // stage stores elements 8 j + r, and the two values of j in a half-wave
+    // (16 lines each) then land in different halves of the bank row
+    auto buf = [&](int i) -> C2<TC> & {
+        const int row = PDO_FFT_STRIDED_SWZ ? (i ^ ((i >> 3) & 1)) : i;
+        return tile[row * TX + ln];
+    };
     if constexpr (MODE == 0) {
         line_fft<TC, -1, false, L>(ldg, stg, buf, tw, jb, act);
     } else if constexpr (MODE == 1) {
@@ -280,14 +293,41 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
     }
 }
 
-// LDS layout of an x line: one pad element after every 8, so the radix-8
-// Stockham stores (8 j + r and 64 m + k + 8 r across the lanes j of a line)
-// fall in distinct banks
-#ifndef PDO_FFT_X_PAD
-#define PDO_FFT_X_PAD 1
+// LDS layout of an x line of L complex points.  An LDS access of 8-byte
+// values is conflict-free when the 32 lanes of each half-wave fall in 32
+// distinct slots (address mod 32).  The Stockham stages access a line
+// contiguously (loads, the split/merge steps) and, in their stores, at
+// 8 j + r and 64 (j / 8) + j % 8 + 8 r across the lanes j.
+//   1: one pad element after every 8 (lines of 128 points or fewer, where a
+//      half-wave spans several lines);
+//   2: (L >= 256, a half-wave within one line) element i of 32-block b at
+//      i ^ ((b & 7) | ((b >> 1) & 3) << 3): a permutation inside each block,
+//      so contiguous runs stay conflict-free, and both store patterns spread
+//      over all 32 slots.
+// 0 is the plain layout.  Tuning constant.
+#ifndef PDO_FFT_X_LAYOUT
+#define PDO_FFT_X_LAYOUT 2
 #endif
-__host__ __device__ constexpr int xpad(int i) { return PDO_FFT_X_PAD ? i + (i >> 3) : i; }
-__host__ __device__ constexpr int xline_pitch(int L) { return xpad(L) + 1; }
+template <int L> __host__ __device__ constexpr int xslot(int i)
+{
+    if constexpr (PDO_FFT_X_LAYOUT == 2 && L >= 256) {
+        const int b = i >> 5;
+        return i ^ ((b & 7) | (((b >> 1) & 3) << 3));
+    } else if constexpr (PDO_FFT_X_LAYOUT >= 1) {
+        return i + (i >> 3);
+    } else {
+        return i;
+    }
+}
+// line pitch (elements): past the last slot of element L; the swizzled
+// layout adds half a bank row so that consecutive lines start 16 slots apart
+template <int L> __host__ __device__ constexpr int xline_pitch()
+{
+    if constexpr (PDO_FFT_X_LAYOUT == 2 && L >= 256)
+        return ((L + 32) & ~31) + 16;
+    else
+        return xslot<L>(L) + 1;
+}
 
 // Z_k = E_k + i O_k of the half-length inverse transform from the staged
 // modes X_k (in place, one thread per pair (k, L-k)):
@@ -342,14 +382,14 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c(const TS *__restrict__ in, T
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
-    constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch(L), nxh = L + 1;
+    constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch<L>(), nxh = L + 1;
     const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
     const long l = (long)blockIdx.x * TXL + line;
     const bool act = (line < TXL) && (l < nlines);
     using IO = CLoad<TS, TC>;
     const TS *src = in + (act ? l : 0) * (long)nx;
     auto ldg = [&](int i) { return IO::ld(src, i); };  // packed z_j = (x_2j, x_2j+1)
-    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xpad(i)]; };
+    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xslot<L>(i)]; };
     auto sts = [&](int i, C2<TC> v) { buf(i) = v; };
     line_fft<TC, -1, false, L>(ldg, sts, buf, (const C2<TC> *)twL, jb, act);
     __syncthreads();
@@ -370,13 +410,13 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r(const TS *__restrict__ in, T
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
-    constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch(L);
+    constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch<L>();
     const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
     const long l = (long)blockIdx.x * TXL + line;
     const bool act = (line < TXL) && (l < nlines);
     using IO = CLoad<TS, TC>;
     const TS *src = in + (act ? l : 0) * (long)nxp * 2;
-    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xpad(i)]; };
+    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xslot<L>(i)]; };
     const C2<TC> *wN = (const C2<TC> *)twN;
     // stage the nx/2+1 modes of the line in LDS (coalesced, all loads of a
     // thread in flight together), then form Z_k = E_k + i O_k in place, one
@@ -421,13 +461,13 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c_dx(const TS *__restrict__ di
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
-    constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch(L), nxh = L + 1;
+    constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch<L>(), nxh = L + 1;
     constexpr int NK = (nxh + L8 - 1) / L8;  // modes per thread
     const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
     const long l = (long)blockIdx.x * TXL + line;
     const bool act = (line < TXL) && (l < nlines);
     using IO = CLoad<TS, TC>;
-    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xpad(i)]; };
+    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xslot<L>(i)]; };
     auto sts = [&](int i, C2<TC> v) { buf(i) = v; };
     const C2<TC> *wN = (const C2<TC> *)twN;
     // i k' FFT(ut), held in registers
@@ -472,13 +512,13 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r_dx(const TS *__restrict__ in
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
-    constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch(L);
+    constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch<L>();
     const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
     const long l = (long)blockIdx.x * TXL + line;
     const bool act = (line < TXL) && (l < nlines);
     using IO = CLoad<TS, TC>;
     const TS *src = in + (act ? l : 0) * (long)nxp * 2;
-    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xpad(i)]; };
+    auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xslot<L>(i)]; };
     auto ldb = [&](int i) { return buf(i); };
     const C2<TC> *wN = (const C2<TC> *)twN;
     C2<TC> v[8];
@@ -588,13 +628,13 @@ static void launch_x_fft(bool inverse, const TS *in, TS *out, long nlines, const
     const size_t ce = 2 * sizeof(TC);
     int TXL = PDO_FFT_X_THREADS / (L / 8);
     if (TXL < 1) TXL = 1;
-    while (TXL > 1 && (size_t)TXL * xline_pitch(L) * ce > 64 * 1024) TXL >>= 1;
+    while (TXL > 1 && (size_t)TXL * xline_pitch<L>() * ce > 64 * 1024) TXL >>= 1;
     dim3 grid((unsigned)((nlines + TXL - 1) / TXL));
     if (!inverse)
-        hipLaunchKernelGGL((k_fft_x_r2c<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch(L) * ce, st, in,
+        hipLaunchKernelGGL((k_fft_x_r2c<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch<L>() * ce, st, in,
                            out, nlines, tw, twn, TXL, nxp);
     else
-        hipLaunchKernelGGL((k_fft_x_c2r<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch(L) * ce, st, in,
+        hipLaunchKernelGGL((k_fft_x_c2r<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch<L>() * ce, st, in,
                            out, nlines, tw, twn, TXL, scale, nxp, oddball);
 }
 
@@ -621,13 +661,13 @@ static void launch_x_fft_dx(bool inverse, const TS *a, const TS *b, TS *o1, TS *
     const size_t ce = 2 * sizeof(TC);
     int TXL = PDO_FFT_X_THREADS / (L / 8);
     if (TXL < 1) TXL = 1;
-    while (TXL > 1 && (size_t)TXL * xline_pitch(L) * ce > 64 * 1024) TXL >>= 1;
+    while (TXL > 1 && (size_t)TXL * xline_pitch<L>() * ce > 64 * 1024) TXL >>= 1;
     dim3 grid((unsigned)((nlines + TXL - 1) / TXL));
     if (!inverse)
-        hipLaunchKernelGGL((k_fft_x_r2c_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch(L) * ce, st, a,
+        hipLaunchKernelGGL((k_fft_x_r2c_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch<L>() * ce, st, a,
                            b, o1, nlines, tw, twn, TXL, nxp, kp);
     else
-        hipLaunchKernelGGL((k_fft_x_c2r_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch(L) * ce, st, a,
+        hipLaunchKernelGGL((k_fft_x_c2r_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch<L>() * ce, st, a,
                            o1, b, o2, nlines, tw, twn, TXL, scale, nxp, kp);
 }
 
