@@ -244,17 +244,16 @@ __device__ __forceinline__ void buf_st(double v, __amdgpu_buffer_rsrc_t r, unsig
 template <int FAM, int MODE, int NPW, bool PER, typename PT>
 __device__ __forceinline__ void reduce_seg(const typename Mode<MODE>::TC *red, const ApplyArgs &A, int s, int pl,
                                            typename Mode<MODE>::TC (&yl)[2], typename Mode<MODE>::TC (&yr)[2],
-                                           PT Gtab)
+                                           PT Gtab, int D)
 {
     using TC = typename Mode<MODE>::TC;
     constexpr int K = Fam<FAM>::K;
     const int P = A.P, NR = 2 * K * P;
     yl[0] = yl[1] = yr[0] = yr[1] = TC(0);
     if constexpr (PER) {
-        // only the segment offsets whose coupling is above the working
-        // precision are summed: dd in [-D, D] (all P when 2D+1 >= P)
+        // only the segment offsets whose coupling is above the solve's
+        // tolerance are summed: dd in [-D, D] (all P when 2D+1 >= P)
         const PT G = Gtab;
-        const int D = (sizeof(TC) == 8) ? A.D64 : A.D32;
         const int nd = (2 * D + 1 < P) ? 2 * D + 1 : P;
         const int d0 = (2 * D + 1 < P) ? P - D : 0;
         // segment t = s + dd (mod P) advances with dd: a running index with
@@ -328,7 +327,9 @@ __device__ __forceinline__ void strided_compute(const ApplyArgs &A, const typena
         red[(s * 2 * K + K + q) * NPW + pl] = d[m - K + q];
     }
     __syncthreads();
-    reduce_seg<FAM, MODE, NPW, PER>(red, A, s, pl, yl, yr, Gs);
+    // coupling reach of the first solve (and of the only one, unrefined)
+    const int D1 = (sizeof(TC) == 8) ? A.D64 : (REFINE ? A.D32R : A.D32);
+    reduce_seg<FAM, MODE, NPW, PER>(red, A, s, pl, yl, yr, Gs, D1);
     apply_spikes<K, MM, true>(d, spk, yl, yr, m);
 
     if constexpr (REFINE) {
@@ -377,7 +378,7 @@ __device__ __forceinline__ void strided_compute(const ApplyArgs &A, const typena
         }
         __syncthreads();
         TC el[2], er[2];
-        reduce_seg<FAM, MODE, NPW, PER>(red2, A, s, pl, el, er, Gs);
+        reduce_seg<FAM, MODE, NPW, PER>(red2, A, s, pl, el, er, Gs, A.D32C);
         apply_spikes<K, MM, true>(e, spk, el, er, m);
 #pragma unroll
         for (int j = 0; j < MM; ++j) d[j] += e[j];
@@ -567,7 +568,7 @@ __device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename 
     // exchange buffers hold value r of lane (p, s) at r * NT + p * P + s: the
     // lanes of a half-wave touch consecutive words (conflict-free)
     const int NT = TP * P;
-    auto exchange = [&](TC *rb, TC (&v)[MM], TC (&yl)[2], TC (&yr)[2]) {
+    auto exchange = [&](TC *rb, TC (&v)[MM], TC (&yl)[2], TC (&yr)[2], int D) {
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             rb[q * NT + p * P + s] = v[q];
@@ -575,7 +576,6 @@ __device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename 
         }
         __syncthreads();
         yl[0] = yl[1] = yr[0] = yr[1] = TC(0);
-        const int D = (sizeof(TC) == 8) ? A.D64 : A.D32;
         const int nd = (2 * D + 1 < P) ? 2 * D + 1 : P;
         const int d0 = (2 * D + 1 < P) ? P - D : 0;
         // periodic systems are block circulant: G_s[.][t] = G_0[.][t - s].
@@ -606,7 +606,7 @@ __device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename 
 
     TC yl[2], yr[2];
     local_solve<K, MM, true>(d, fac, m);
-    exchange(red, d, yl, yr);
+    exchange(red, d, yl, yr, (sizeof(TC) == 8) ? A.D64 : (REFINE ? A.D32R : A.D32));
     apply_spikes<K, MM, true>(d, spk, yl, yr, m);
     if constexpr (REFINE) {
         // neighbour rows of the residual from the values the neighbours hold
@@ -643,7 +643,7 @@ __device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename 
         local_solve<K, MM, true>(e, fac, m);
         TC el[2], er[2];
         if (PDO_XDIR_RED_SLOTS == 1) __syncthreads();  // the neighbour reads of the slot are done
-        exchange(red + (size_t)(PDO_XDIR_RED_SLOTS == 3 ? 1 : 0) * TP * P * 2 * K, e, el, er);
+        exchange(red + (size_t)(PDO_XDIR_RED_SLOTS == 3 ? 1 : 0) * TP * P * 2 * K, e, el, er, A.D32C);
         apply_spikes<K, MM, true>(e, spk, el, er, m);
 #pragma unroll
         for (int j = 0; j < MM; ++j) d[j] += e[j];

@@ -28,6 +28,7 @@ struct ApplyArgs {
     int m, P;
     int lgn;           // log2(n_in) (x kernel; n_in is a power of two there)
     int D32, D64;      // circulant coupling offsets kept by FP32 / FP64 solves
+    int D32R, D32C;    // ... by the refined mode's first solve / its correction
     int nbl, nbr, wb;
     const double *tab_d;
     const float *tab_f;

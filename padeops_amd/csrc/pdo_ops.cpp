@@ -169,7 +169,7 @@ static int prepare(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, 
     A.n_sys = s.n_sys; A.n_in = s.n_in; A.n_out = s.n_out;
     A.n_read = (s.family == FAM_STG_E2C_A || s.family == FAM_STG_E2C_S) ? s.n_sys + 1 : s.n_sys;
     A.m = t.m; A.P = t.P; A.nbl = t.nbl; A.nbr = t.nbr; A.wb = t.wb;
-    A.D32 = t.D32; A.D64 = t.D64;
+    A.D32 = t.D32; A.D64 = t.D64; A.D32R = t.D32R; A.D32C = t.D32C;
     A.lgn = 0;
     while ((1 << A.lgn) < s.n_in) A.lgn++;
     A.tab_d = ts->d_tab; A.tab_f = ts->f_tab;

@@ -786,7 +786,7 @@ int build_spike(const SystemDesc &s, SpikeTables &t, bool force_general)
     }
     invert(R, NR);
     // decay of the (block circulant, periodic) coupling with segment offset
-    t.D32 = t.D64 = P;
+    t.D32 = t.D64 = t.D32R = t.D32C = P;
     if (s.periodic && P > 1) {
         std::vector<double> blk(P, 0.0);
         for (int q = 0; q < 2 * K; q++)
@@ -802,6 +802,8 @@ int build_spike(const SystemDesc &s, SpikeTables &t, bool force_general)
         };
         t.D32 = reach(1e-9);
         t.D64 = reach(1e-18);
+        t.D32R = reach(1e-6);
+        t.D32C = reach(1e-3);
     }
     for (int sg = 0; sg < P; sg++) {
         const int pv = prev_of(sg), nx = next_of(sg);

@@ -87,6 +87,11 @@ struct SpikeTables {
     std::vector<double> G;     // [P][2K][2K*P]: rows of R^-1 giving yL[0..K), yR[0..K)
     std::vector<double> sp;    // [2][NBMAX][WBMAX]: closure rows (left: cols 0..WBMAX-1, right: cols n_in-WBMAX..n_in-1)
     int D32 = 0, D64 = 0;      // periodic: segment offsets +-D whose coupling exceeds 1e-9 / 1e-18 of the largest
+    // the refined FP32 mode: its first solve keeps couplings above 1e-6 (its
+    // error is removed by the correction), the correction those above 1e-3
+    // (the correction is ~1e-5 of the solution, so its own relative error
+    // of 1e-3 leaves ~1e-8)
+    int D32R = 0, D32C = 0;
 };
 
 int build_spike(const SystemDesc &s, SpikeTables &t, bool force_general = false);
