@@ -213,6 +213,17 @@ template <> struct CLoad<float, double> : CLoadF<double> {};
 __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// twiddles of the strided transforms staged in LDS (1) or read from the
+// table in HBM / cache (0); tuning constant
+#ifndef PDO_FFT_TW_LDS
+#define PDO_FFT_TW_LDS 1
+#endif
+
+// the same for the x-line transforms; tuning constant
+#ifndef PDO_FFT_X_TW_LDS
+#define PDO_FFT_X_TW_LDS PDO_FFT_TW_LDS
+#endif
+
 // LDS rows of the strided transforms swizzled (1) or plain (0); tuning constant
 #ifndef PDO_FFT_STRIDED_SWZ
 #define PDO_FFT_STRIDED_SWZ 1
@@ -239,16 +250,24 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
     const TS *const lin = src + off;
     TS *const lout = dst + off;
     const C2<TC> *tw = (const C2<TC> *)twL;
+    if constexpr (PDO_FFT_TW_LDS) {
+        // the twiddle table (L values) in LDS after the tile: every stage
+        // reads it with broadcast LDS loads instead of vector memory loads
+        C2<TC> *twl = tile + TX * L;
+        for (int i = threadIdx.x; i < L; i += TX * (L / 8)) twl[i] = tw[i];
+        __syncthreads();
+        tw = twl;
+    }
     using IO = CLoad<TS, TC>;
     auto ldg = [&](int i) { return IO::ld(lin, i * se); };
     auto stg = [&](int i, C2<TC> v) { IO::st(lout, i * se, v); };
-    // element i of the line at row i ^ ((i >> 3) & 1): the first Stockham
-    // stage stores elements 8 j + r, and the two values of j in a half-wave
-    // (16 lines each) then land in different halves of the bank row
-    auto buf = [&](int i) -> C2<TC> & {
-        const int row = PDO_FFT_STRIDED_SWZ ? (i ^ ((i >> 3) & 1)) : i;
-        return tile[row * TX + ln];
-    };
+    // element i of the line at row i ^ ((i >> 3) & SW): a half-wave spans
+    // SW + 1 = 256 B / (TX complex) rows, and the first Stockham stage stores
+    // rows 8 j + r, one per value of j in the half-wave; the swizzle sends
+    // them to distinct parts of the 256-byte bank row
+    constexpr int SLOTS = 256 / (int)sizeof(C2<TC>);
+    constexpr int SW = (PDO_FFT_STRIDED_SWZ && SLOTS > TX) ? (SLOTS / TX - 1 < 7 ? SLOTS / TX - 1 : 7) : 0;
+    auto buf = [&](int i) -> C2<TC> & { return tile[(i ^ ((i >> 3) & SW)) * TX + ln]; };
     if constexpr (MODE == 0) {
         line_fft<TC, -1, false, L>(ldg, stg, buf, tw, jb, act);
     } else if constexpr (MODE == 1) {
@@ -319,14 +338,36 @@ template <int L> __host__ __device__ constexpr int xslot(int i)
         return i;
     }
 }
-// line pitch (elements): past the last slot of element L; the swizzled
-// layout adds half a bank row so that consecutive lines start 16 slots apart
+// line pitch (elements): past the last slot of element L (with L >= 256 a
+// half-wave never spans two lines, so the lines need no relative offset)
 template <int L> __host__ __device__ constexpr int xline_pitch()
 {
     if constexpr (PDO_FFT_X_LAYOUT == 2 && L >= 256)
-        return ((L + 32) & ~31) + 16;
+        return (L + 32) & ~31;
     else
         return xslot<L>(L) + 1;
+}
+
+// LDS bytes of an x-line workgroup: TXL lines and the staged twiddles
+template <int L> static size_t xline_lds(int TXL, size_t ce)
+{
+    return ((size_t)TXL * xline_pitch<L>() + (PDO_FFT_X_TW_LDS ? L : 0)) * ce;
+}
+
+// the twiddle table of an x line (L values) staged in LDS after the tile of
+// TXL lines; returns the LDS copy (or the table itself when staging is off).
+// The first use is in the second Stockham stage, after a barrier.
+template <typename TC, int L>
+__device__ __forceinline__ const C2<TC> *x_twiddles(C2<TC> *tile, int TXL, const TC *twL)
+{
+    if constexpr (PDO_FFT_X_TW_LDS) {
+        C2<TC> *twl = tile + (size_t)TXL * xline_pitch<L>();
+        const C2<TC> *tw = (const C2<TC> *)twL;
+        for (int i = threadIdx.x; i < L; i += blockDim.x) twl[i] = tw[i];
+        return twl;
+    } else {
+        return (const C2<TC> *)twL;
+    }
 }
 
 // Z_k = E_k + i O_k of the half-length inverse transform from the staged
@@ -382,6 +423,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c(const TS *__restrict__ in, T
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
+    const C2<TC> *twl = x_twiddles<TC, L>(tile, TXL, twL);
     constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch<L>(), nxh = L + 1;
     const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
     const long l = (long)blockIdx.x * TXL + line;
@@ -391,7 +433,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c(const TS *__restrict__ in, T
     auto ldg = [&](int i) { return IO::ld(src, i); };  // packed z_j = (x_2j, x_2j+1)
     auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xslot<L>(i)]; };
     auto sts = [&](int i, C2<TC> v) { buf(i) = v; };
-    line_fft<TC, -1, false, L>(ldg, sts, buf, (const C2<TC> *)twL, jb, act);
+    line_fft<TC, -1, false, L>(ldg, sts, buf, twl, jb, act);
     __syncthreads();
     const C2<TC> *wN = (const C2<TC> *)twN;
     TS *dst = out + (act ? l : 0) * (long)nxp * 2;
@@ -410,6 +452,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r(const TS *__restrict__ in, T
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
+    const C2<TC> *twl = x_twiddles<TC, L>(tile, TXL, twL);
     constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch<L>();
     const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
     const long l = (long)blockIdx.x * TXL + line;
@@ -442,7 +485,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r(const TS *__restrict__ in, T
         v.y *= scale;
         IO::st(dst, i, v);
     };
-    line_fft<TC, +1, true, L>(ldb, stg, buf, (const C2<TC> *)twL, jb, act);
+    line_fft<TC, +1, true, L>(ldb, stg, buf, twl, jb, act);
 }
 
 // The x transforms of the full-RHS Poisson solve with the x derivatives of
@@ -461,6 +504,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c_dx(const TS *__restrict__ di
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
+    const C2<TC> *twl = x_twiddles<TC, L>(tile, TXL, twL);
     constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch<L>(), nxh = L + 1;
     constexpr int NK = (nxh + L8 - 1) / L8;  // modes per thread
     const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
@@ -475,7 +519,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c_dx(const TS *__restrict__ di
     {
         const TS *src = ut + (act ? l : 0) * (long)nx;
         auto ldg = [&](int i) { return IO::ld(src, i); };
-        line_fft<TC, -1, false, L>(ldg, sts, buf, (const C2<TC> *)twL, jb, act);
+        line_fft<TC, -1, false, L>(ldg, sts, buf, twl, jb, act);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < NK; ++q) {
@@ -491,7 +535,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c_dx(const TS *__restrict__ di
     }
     const TS *src = div + (act ? l : 0) * (long)nx;
     auto ldg = [&](int i) { return IO::ld(src, i); };
-    line_fft<TC, -1, false, L>(ldg, sts, buf, (const C2<TC> *)twL, jb, act);
+    line_fft<TC, -1, false, L>(ldg, sts, buf, twl, jb, act);
     __syncthreads();
     TS *dst = out + (act ? l : 0) * (long)nxp * 2;
     if (act) {
@@ -512,6 +556,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r_dx(const TS *__restrict__ in
 {
     extern __shared__ unsigned char smem[];
     C2<TC> *tile = (C2<TC> *)smem;
+    const C2<TC> *twl = x_twiddles<TC, L>(tile, TXL, twL);
     constexpr int nx = 2 * L, L8 = L >> 3, LDL = xline_pitch<L>();
     const int line = threadIdx.x / L8, jb = threadIdx.x - line * L8;
     const long l = (long)blockIdx.x * TXL + line;
@@ -540,7 +585,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r_dx(const TS *__restrict__ in
         x.y *= scale;
         IO::st(pl, i, x);
     };
-    line_fft<TC, +1, true, L>(ldb, stp, buf, (const C2<TC> *)twL, jb, act);
+    line_fft<TC, +1, true, L>(ldb, stp, buf, twl, jb, act);
     __syncthreads();  // every read of the buffer by the first transform is done
     // i k' P, from the modes still in registers
     if (act) {
@@ -563,7 +608,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r_dx(const TS *__restrict__ in
         const C2<TC> a = IO::ld(ul, i);
         IO::st(uo, i, C2<TC>{a.x - scale * x.x, a.y - scale * x.y});
     };
-    line_fft<TC, +1, true, L>(ldb, stu, buf, (const C2<TC> *)twL, jb, act);
+    line_fft<TC, +1, true, L>(ldb, stu, buf, twl, jb, act);
 }
 
 // ---------------------------------------------------------------------------
@@ -584,11 +629,18 @@ static int ilog2(int v)
 #define PDO_FFT_X_THREADS 1024
 #endif
 
+// the fused forward-multiply-inverse pass (MODE 2) may use smaller
+// workgroups than the single transforms; tuning constant
+#ifndef PDO_FFT_FUSED_THREADS
+#define PDO_FFT_FUSED_THREADS 1024
+#endif
+
 // lines per workgroup of the strided transforms: PDO_FFT_STRIDED_THREADS
-// threads (8 points each), at most 64 KiB of LDS
-template <int L, size_t CE> constexpr int strided_fft_lines()
+// (PDO_FFT_FUSED_THREADS for MODE 2) threads, 8 points each, at most 64 KiB
+// of LDS
+template <int L, size_t CE, int MODE> constexpr int strided_fft_lines()
 {
-    int tx = PDO_FFT_STRIDED_THREADS / (L / 8);
+    int tx = (MODE == 2 ? PDO_FFT_FUSED_THREADS : PDO_FFT_STRIDED_THREADS) / (L / 8);
     if (tx < 1) tx = 1;
     while (tx > 1 && (size_t)tx * L * CE > 64 * 1024) tx >>= 1;
     return tx;
@@ -599,9 +651,10 @@ static void launch_strided_fft(const TS *src, TS *dst, long na, long sb, long se
                                int nb, hipStream_t st)
 {
     constexpr size_t ce = 2 * sizeof(TC);
-    constexpr int TX = strided_fft_lines<L, ce>();
+    constexpr int TX = strided_fft_lines<L, ce, MODE>();
     dim3 grid((unsigned)((na + TX - 1) / TX), (unsigned)nb);
-    hipLaunchKernelGGL((k_fft_strided<TS, TC, MODE, L, TX>), grid, dim3(TX * (L / 8)), (size_t)TX * L * ce, st, src,
+    hipLaunchKernelGGL((k_fft_strided<TS, TC, MODE, L, TX>), grid, dim3(TX * (L / 8)),
+                       (size_t)(TX + (PDO_FFT_TW_LDS ? 1 : 0)) * L * ce, st, src,
                        dst, na, sb, (int)se, tw, pm);
 }
 
@@ -631,10 +684,10 @@ static void launch_x_fft(bool inverse, const TS *in, TS *out, long nlines, const
     while (TXL > 1 && (size_t)TXL * xline_pitch<L>() * ce > 64 * 1024) TXL >>= 1;
     dim3 grid((unsigned)((nlines + TXL - 1) / TXL));
     if (!inverse)
-        hipLaunchKernelGGL((k_fft_x_r2c<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch<L>() * ce, st, in,
+        hipLaunchKernelGGL((k_fft_x_r2c<TS, TC, L>), grid, dim3(TXL * (L / 8)), xline_lds<L>(TXL, ce), st, in,
                            out, nlines, tw, twn, TXL, nxp);
     else
-        hipLaunchKernelGGL((k_fft_x_c2r<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch<L>() * ce, st, in,
+        hipLaunchKernelGGL((k_fft_x_c2r<TS, TC, L>), grid, dim3(TXL * (L / 8)), xline_lds<L>(TXL, ce), st, in,
                            out, nlines, tw, twn, TXL, scale, nxp, oddball);
 }
 
@@ -664,10 +717,10 @@ static void launch_x_fft_dx(bool inverse, const TS *a, const TS *b, TS *o1, TS *
     while (TXL > 1 && (size_t)TXL * xline_pitch<L>() * ce > 64 * 1024) TXL >>= 1;
     dim3 grid((unsigned)((nlines + TXL - 1) / TXL));
     if (!inverse)
-        hipLaunchKernelGGL((k_fft_x_r2c_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch<L>() * ce, st, a,
+        hipLaunchKernelGGL((k_fft_x_r2c_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), xline_lds<L>(TXL, ce), st, a,
                            b, o1, nlines, tw, twn, TXL, nxp, kp);
     else
-        hipLaunchKernelGGL((k_fft_x_c2r_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), (size_t)TXL * xline_pitch<L>() * ce, st, a,
+        hipLaunchKernelGGL((k_fft_x_c2r_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), xline_lds<L>(TXL, ce), st, a,
                            o1, b, o2, nlines, tw, twn, TXL, scale, nxp, kp);
 }
 

@@ -128,6 +128,25 @@ def test_fp32_modes(cuda, scheme, meth, oop, dir, prec, bar, periodic):
     assert err <= bar
 
 
+@pytest.mark.parametrize("dir", [1, 2, 3])
+@pytest.mark.parametrize("scheme,meth,oop", COLLOCATED)
+def test_fp32_refine_long_pencils(cuda, scheme, meth, oop, dir):
+    """Periodic n = 512: 16 segments per pencil, so the refined mode's
+    truncated segment couplings (first solve and correction) are in play;
+    white-noise input, the worst case for the cf90 filter's near-singular
+    Nyquist mode."""
+    import torch
+    n = 512
+    f = random_field(_shape(n, dir, (32, 2)), seed=31 + dir).astype(np.float32).astype(np.float64)
+    o = _make(scheme, n, True)
+    extra = () if scheme == "cd06" else (0, 0)
+    g = _run(o, "%s%d" % (meth, dir), f, dir, torch.float32, "f32_refine", cuda, extra)
+    ref = O.apply(oop, f, dir, DX if scheme != "cf90" else 1.0, True)
+    err = rel_l2(g, ref)
+    print("f32_refine n=512", scheme, meth, "dir", dir, "rel_l2", err)
+    assert err <= 2e-7
+
+
 STAGG = [("ddz_E2C", O.STG_D1_E2C, 1, 0), ("ddz_C2E", O.STG_D1_C2E, 0, 1), ("d2dz2_C2C", O.STG_D2_C2C, 0, 0),
          ("d2dz2_E2E", O.STG_D2_E2E, 1, 1), ("InterpZ_E2C", O.STG_IN_E2C, 1, 0),
          ("InterpZ_C2E", O.STG_IN_C2E, 0, 1)]
