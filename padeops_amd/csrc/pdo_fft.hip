@@ -348,10 +348,11 @@ template <int L> __host__ __device__ constexpr int xline_pitch()
         return xslot<L>(L) + 1;
 }
 
-// LDS bytes of an x-line workgroup: TXL lines and the staged twiddles
-template <int L> static size_t xline_lds(int TXL, size_t ce)
+// LDS bytes of an x-line workgroup: TXL lines and the staged twiddles (with
+// split: also the L + 1 split/merge twiddles of the spectral-x kernels)
+template <int L> static size_t xline_lds(int TXL, size_t ce, bool split = false)
 {
-    return ((size_t)TXL * xline_pitch<L>() + (PDO_FFT_X_TW_LDS ? L : 0)) * ce;
+    return ((size_t)TXL * xline_pitch<L>() + (PDO_FFT_X_TW_LDS ? L + (split ? L + 1 : 0) : 0)) * ce;
 }
 
 // the twiddle table of an x line (L values) staged in LDS after the tile of
@@ -367,6 +368,21 @@ __device__ __forceinline__ const C2<TC> *x_twiddles(C2<TC> *tile, int TXL, const
         return twl;
     } else {
         return (const C2<TC> *)twL;
+    }
+}
+
+// the split/merge twiddles (L + 1 values) of the spectral-x kernels staged in
+// LDS after the line twiddles; their first use follows a barrier
+template <typename TC, int L>
+__device__ __forceinline__ const C2<TC> *x_split_twiddles(C2<TC> *tile, int TXL, const TC *twN)
+{
+    if constexpr (PDO_FFT_X_TW_LDS) {
+        C2<TC> *wl = tile + (size_t)TXL * xline_pitch<L>() + L;
+        const C2<TC> *w = (const C2<TC> *)twN;
+        for (int i = threadIdx.x; i <= L; i += blockDim.x) wl[i] = w[i];
+        return wl;
+    } else {
+        return (const C2<TC> *)twN;
     }
 }
 
@@ -513,7 +529,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c_dx(const TS *__restrict__ di
     using IO = CLoad<TS, TC>;
     auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xslot<L>(i)]; };
     auto sts = [&](int i, C2<TC> v) { buf(i) = v; };
-    const C2<TC> *wN = (const C2<TC> *)twN;
+    const C2<TC> *wN = x_split_twiddles<TC, L>(tile, TXL, twN);
     // i k' FFT(ut), held in registers
     C2<TC> du[NK];
     {
@@ -565,7 +581,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r_dx(const TS *__restrict__ in
     const TS *src = in + (act ? l : 0) * (long)nxp * 2;
     auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xslot<L>(i)]; };
     auto ldb = [&](int i) { return buf(i); };
-    const C2<TC> *wN = (const C2<TC> *)twN;
+    const C2<TC> *wN = x_split_twiddles<TC, L>(tile, TXL, twN);
     C2<TC> v[8];
     C2<TC> vl = {};
     if (act) {
@@ -717,10 +733,10 @@ static void launch_x_fft_dx(bool inverse, const TS *a, const TS *b, TS *o1, TS *
     while (TXL > 1 && (size_t)TXL * xline_pitch<L>() * ce > 64 * 1024) TXL >>= 1;
     dim3 grid((unsigned)((nlines + TXL - 1) / TXL));
     if (!inverse)
-        hipLaunchKernelGGL((k_fft_x_r2c_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), xline_lds<L>(TXL, ce), st, a,
+        hipLaunchKernelGGL((k_fft_x_r2c_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), xline_lds<L>(TXL, ce, true), st, a,
                            b, o1, nlines, tw, twn, TXL, nxp, kp);
     else
-        hipLaunchKernelGGL((k_fft_x_c2r_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), xline_lds<L>(TXL, ce), st, a,
+        hipLaunchKernelGGL((k_fft_x_c2r_dx<TS, TC, L>), grid, dim3(TXL * (L / 8)), xline_lds<L>(TXL, ce, true), st, a,
                            o1, b, o2, nlines, tw, twn, TXL, scale, nxp, kp);
 }
 
