@@ -17,6 +17,12 @@
 #ifndef PDO_RHS_SIDE_STREAM
 #define PDO_RHS_SIDE_STREAM 1
 #endif
+// gradient components (of 3) launched beside the filter chain; the others
+// run beside the divergence, the Poisson solve and the projection; tuning
+// constant
+#ifndef PDO_RHS_GRAD_EARLY
+#define PDO_RHS_GRAD_EARLY 3
+#endif
 // form the x terms of the divergence and of the projection inside the
 // Poisson solve's x transforms (1) or as compact x passes (0)
 #ifndef PDO_RHS_SPECTRAL_X
@@ -39,10 +45,11 @@ struct RhsObject {
     // the nine gradient passes run on a second stream, concurrently with the
     // filter chain; fork and join are events on the caller's stream
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_fork2 = nullptr, ev_join = nullptr;
     ~RhsObject()
     {
         if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_fork2) (void)hipEventDestroy(ev_fork2);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (side) (void)hipStreamDestroy(side);
         for (void *p : ut) if (p) (void)hipFree(p);
@@ -81,6 +88,7 @@ int pdo_rhs_init(pdo_rhs *h, int n, double L, int prec)
     if (e == hipSuccess) e = hipMalloc(&o->partial, o->nblocks * sizeof(double));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&o->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&o->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&o->ev_fork2, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&o->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) { delete o; return hip_fail(e, "hipMalloc(rhs work buffers)"); }
     *h = o;
@@ -105,22 +113,32 @@ int pdo_rhs_eval(pdo_rhs h, const void *u, const void *v, const void *w, void *g
     auto op = [&](OpObject *o, int kind, int d, const void *f, void *out, int epi, const void *aux) {
         return op_on(st, o, kind, d, f, out, epi, aux);
     };
-    // 9 velocity-gradient components, on the side stream (they only read u_i)
-    hipError_t e = hipEventRecord(h->ev_fork, st);
-    if (e == hipSuccess) e = hipStreamWaitEvent(h->side, h->ev_fork, 0);
-    if (e != hipSuccess) return hip_fail(e, "stream fork");
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++)
-            if ((r = op_on(PDO_RHS_SIDE_STREAM ? h->side : st, &h->d1, KIND_D1, j + 1, ui[i], h->field(grad, 3 * i + j),
-                           0, nullptr)))
-                return r;
-    if ((e = hipEventRecord(h->ev_join, h->side)) != hipSuccess) return hip_fail(e, "stream join");
+    // the 9 velocity-gradient components run on the side stream (they only
+    // read u_i): those of the first PDO_RHS_GRAD_EARLY components beside the
+    // filter chain, the others beside the divergence / Poisson / projection
+    hipError_t e = hipSuccess;
+    auto gradients = [&](int i0, int i1, hipEvent_t fork) -> int {
+        if (i0 >= i1) return 0;
+        hipError_t ee = hipEventRecord(fork, st);
+        if (ee == hipSuccess) ee = hipStreamWaitEvent(h->side, fork, 0);
+        if (ee != hipSuccess) return hip_fail(ee, "stream fork");
+        for (int i = i0; i < i1; i++)
+            for (int j = 0; j < 3; j++) {
+                const int rr = op_on(PDO_RHS_SIDE_STREAM ? h->side : st, &h->d1, KIND_D1, j + 1, ui[i],
+                                     h->field(grad, 3 * i + j), 0, nullptr);
+                if (rr) return rr;
+            }
+        return 0;
+    };
+    if ((r = gradients(0, PDO_RHS_GRAD_EARLY, h->ev_fork))) return r;
     // filtered velocity ut_i = F_z F_y F_x u_i (uo_i is the x-pass scratch)
     for (int i = 0; i < 3; i++) {
         if ((r = op(&h->fil, KIND_FILTER, 1, ui[i], uoi[i], 0, nullptr))) return r;
         if ((r = op(&h->fil, KIND_FILTER, 2, uoi[i], h->tmp, 0, nullptr))) return r;
         if ((r = op(&h->fil, KIND_FILTER, 3, h->tmp, h->ut[i], 0, nullptr))) return r;
     }
+    if ((r = gradients(PDO_RHS_GRAD_EARLY, 3, h->ev_fork2))) return r;
+    if ((e = hipEventRecord(h->ev_join, h->side)) != hipSuccess) return hip_fail(e, "stream join");
     // divergence of the filtered velocity, accumulated in place
     if (PDO_RHS_SPECTRAL_X) {
         // the z and y terms as compact passes; the x term, and the x
