@@ -250,11 +250,18 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
     const TS *const lin = src + off;
     TS *const lout = dst + off;
     const C2<TC> *tw = (const C2<TC> *)twL;
+    const TC *azl = nullptr;
     if constexpr (PDO_FFT_TW_LDS) {
         // the twiddle table (L values) in LDS after the tile: every stage
-        // reads it with broadcast LDS loads instead of vector memory loads
+        // reads it with broadcast LDS loads instead of vector memory loads;
+        // the fused pass also stages its per-element factors az (L values)
         C2<TC> *twl = tile + TX * L;
         for (int i = threadIdx.x; i < L; i += TX * (L / 8)) twl[i] = tw[i];
+        if constexpr (MODE == 2) {
+            TC *a = (TC *)(twl + L);
+            for (int i = threadIdx.x; i < L; i += TX * (L / 8)) a[i] = (TC)pm.az[i];
+            azl = a;
+        }
         __syncthreads();
         tw = twl;
     }
@@ -284,7 +291,7 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
         const TC scale = (TC)pm.scale;
         auto sts = [&](int i, C2<TC> v) {
             TC fac;
-            const TC az = (TC)pm.az[i];
+            const TC az = PDO_FFT_TW_LDS ? azl[i] : (TC)pm.az[i];
             if (pm.kind == 3) {
                 // i az: (x + i y) i a = -a y + i a x
                 const TC s = az * scale;
@@ -352,7 +359,8 @@ template <int L> __host__ __device__ constexpr int xline_pitch()
 // split: also the L + 1 split/merge twiddles of the spectral-x kernels)
 template <int L> static size_t xline_lds(int TXL, size_t ce, bool split = false)
 {
-    return ((size_t)TXL * xline_pitch<L>() + (PDO_FFT_X_TW_LDS ? L + (split ? L + 1 : 0) : 0)) * ce;
+    // with split, also the L + 1 wavenumbers (one value each, half a complex)
+    return ((size_t)TXL * xline_pitch<L>() + (PDO_FFT_X_TW_LDS ? L + (split ? L + 1 + (L + 2) / 2 : 0) : 0)) * ce;
 }
 
 // the twiddle table of an x line (L values) staged in LDS after the tile of
@@ -383,6 +391,20 @@ __device__ __forceinline__ const C2<TC> *x_split_twiddles(C2<TC> *tile, int TXL,
         return wl;
     } else {
         return (const C2<TC> *)twN;
+    }
+}
+
+// the modified wavenumbers k'(kx) (L + 1 values) of the spectral-x kernels
+// staged in LDS, in the transform's arithmetic, after the split twiddles
+template <typename TC, int L>
+__device__ __forceinline__ const TC *x_wavenumbers(C2<TC> *tile, int TXL, const double *kp)
+{
+    if constexpr (PDO_FFT_X_TW_LDS) {
+        TC *kl = (TC *)(tile + (size_t)TXL * xline_pitch<L>() + 2 * L + 1);
+        for (int i = threadIdx.x; i <= L; i += blockDim.x) kl[i] = (TC)kp[i];
+        return kl;
+    } else {
+        return nullptr;
     }
 }
 
@@ -530,6 +552,8 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c_dx(const TS *__restrict__ di
     auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xslot<L>(i)]; };
     auto sts = [&](int i, C2<TC> v) { buf(i) = v; };
     const C2<TC> *wN = x_split_twiddles<TC, L>(tile, TXL, twN);
+    const TC *kl = x_wavenumbers<TC, L>(tile, TXL, kp);
+    auto kpr = [&](int k) { return PDO_FFT_X_TW_LDS ? kl[k] : (TC)kp[k]; };
     // i k' FFT(ut), held in registers
     C2<TC> du[NK];
     {
@@ -543,7 +567,7 @@ __global__ void __launch_bounds__(1024) k_fft_x_r2c_dx(const TS *__restrict__ di
             du[q] = C2<TC>{0, 0};
             if (act && k < nxh) {
                 const C2<TC> x = r2c_split<TC, L>(buf, wN, k);
-                const TC kk = (TC)kp[k];
+                const TC kk = kpr(k);
                 du[q] = C2<TC>{-kk * x.y, kk * x.x};
             }
         }
@@ -582,6 +606,8 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r_dx(const TS *__restrict__ in
     auto buf = [&](int i) -> C2<TC> & { return tile[line * LDL + xslot<L>(i)]; };
     auto ldb = [&](int i) { return buf(i); };
     const C2<TC> *wN = x_split_twiddles<TC, L>(tile, TXL, twN);
+    const TC *kl = x_wavenumbers<TC, L>(tile, TXL, kp);
+    auto kpr = [&](int k) { return PDO_FFT_X_TW_LDS ? kl[k] : (TC)kp[k]; };
     C2<TC> v[8];
     C2<TC> vl = {};
     if (act) {
@@ -607,11 +633,11 @@ __global__ void __launch_bounds__(1024) k_fft_x_c2r_dx(const TS *__restrict__ in
     if (act) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const TC kk = (TC)kp[jb + q * L8];
+            const TC kk = kpr(jb + q * L8);
             buf(jb + q * L8) = C2<TC>{-kk * v[q].y, kk * v[q].x};
         }
         if (jb == 0) {
-            const TC kk = (TC)kp[L];
+            const TC kk = kpr(L);
             buf(L) = C2<TC>{-kk * vl.y, kk * vl.x};
         }
     }
@@ -670,7 +696,9 @@ static void launch_strided_fft(const TS *src, TS *dst, long na, long sb, long se
     constexpr int TX = strided_fft_lines<L, ce, MODE>();
     dim3 grid((unsigned)((na + TX - 1) / TX), (unsigned)nb);
     hipLaunchKernelGGL((k_fft_strided<TS, TC, MODE, L, TX>), grid, dim3(TX * (L / 8)),
-                       (size_t)(TX + (PDO_FFT_TW_LDS ? 1 : 0)) * L * ce, st, src,
+                       (size_t)(TX + (PDO_FFT_TW_LDS ? 1 : 0)) * L * ce +
+                           ((PDO_FFT_TW_LDS && MODE == 2) ? (size_t)L * sizeof(TC) : 0),
+                       st, src,
                        dst, na, sb, (int)se, tw, pm);
 }
 
