@@ -668,7 +668,7 @@ static int ilog2(int v)
 #define PDO_FFT_STRIDED_THREADS 1024
 #endif
 #ifndef PDO_FFT_X_THREADS
-#define PDO_FFT_X_THREADS 1024
+#define PDO_FFT_X_THREADS 256
 #endif
 
 // the fused forward-multiply-inverse pass (MODE 2) may use smaller
