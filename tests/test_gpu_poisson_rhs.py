@@ -66,11 +66,11 @@ def test_poisson_analytic(cuda):
     assert np.abs(g - f).max() < 1e-12
 
 
+@pytest.mark.parametrize("shape", [(64, 64, 64), (512, 16, 8), (1024, 8, 8)])
 @pytest.mark.parametrize("prec,bar", [("f32_refine", 3e-6), ("f32_store", 3e-7)])
-def test_poisson_fp32(cuda, prec, bar):
+def test_poisson_fp32(cuda, prec, bar, shape):
     import torch
-    shape = (64, 64, 64)
-    dx = 2 * np.pi / 64
+    dx = 2 * np.pi / shape[0]
     rhs = random_field(shape, seed=9).astype(np.float32).astype(np.float64)
     ps = P.PoissonPeriodic()
     assert ps.init(dx, dx, dx, *shape, ("cd06",) * 3, 1) == 0
@@ -80,7 +80,7 @@ def test_poisson_fp32(cuda, prec, bar):
     torch.cuda.synchronize()
     ref = O.poisson_periodic(rhs, dx, dx, dx, 1, 1)
     err = rel_l2(from_dev(out, shape), ref)
-    print("poisson", prec, "rel_l2", err)
+    print("poisson", prec, shape, "rel_l2", err)
     assert err <= bar
 
 
