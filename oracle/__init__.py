@@ -29,26 +29,35 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+def _setup(L):
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.orc_apply.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+    L.orc_dense.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                            ctypes.c_int, dp, dp]
+    L.orc_fft_c2c_dir.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int]
+    L.orc_poisson_periodic.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_int] * 2 + [dp, dp]
+    L.orc_modk.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
+    L.orc_full_rhs.argtypes = [ctypes.c_int, ctypes.c_double] + [dp] * 8
+    L.orc_num_threads.restype = ctypes.c_int
+    L.orc_stagg_np.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, dp, dp]
+    L.orc_stagg_np_dense.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp]
+    return L
+
+
+def load(path):
+    """A separately built copy of the oracle library (e.g. a deliberately
+    perturbed build used to show that the pins detect a wrong constant)."""
+    return _setup(ctypes.CDLL(path))
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB):
             build()
-        L = ctypes.CDLL(_LIB)
-        dp = ctypes.POINTER(ctypes.c_double)
-        L.orc_apply.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
-        L.orc_dense.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                                ctypes.c_int, dp, dp]
-        L.orc_fft_c2c_dir.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int]
-        L.orc_poisson_periodic.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_int] * 2 + [dp, dp]
-        L.orc_modk.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
-        L.orc_full_rhs.argtypes = [ctypes.c_int, ctypes.c_double] + [dp] * 8
-        L.orc_num_threads.restype = ctypes.c_int
-        L.orc_stagg_np.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                                   ctypes.c_int, dp, dp]
-        L.orc_stagg_np_dense.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp]
-        _lib = L
+        _lib = _setup(ctypes.CDLL(_LIB))
     return _lib
 
 
@@ -98,12 +107,12 @@ def apply1d(op, f, dx, periodic=True, bc1=0, bcn=0):
     return apply(op, f.reshape(-1, 1, 1, order="F"), 1, dx, periodic, bc1, bcn).reshape(-1)
 
 
-def dense(op, n, dx, periodic=True, bc1=0, bcn=0):
+def dense(op, n, dx, periodic=True, bc1=0, bcn=0, library=None):
     """(A, B): the oracle's LHS matrix (n x n) and RHS stencil matrix (n x n_in)."""
     nin, _ = _io_len(op, n)
     A = np.zeros((n, n))
     B = np.zeros((n, nin))
-    ierr = lib().orc_dense(op, n, dx, int(periodic), bc1, bcn, _p(A), _p(B))
+    ierr = (library or lib()).orc_dense(op, n, dx, int(periodic), bc1, bcn, _p(A), _p(B))
     if ierr:
         raise OracleError(ierr)
     return A, B
@@ -169,11 +178,11 @@ def stagg_np(op, f, dz, flags, n):
     return out
 
 
-def stagg_np_dense(op, n, dz, flags):
+def stagg_np_dense(op, n, dz, flags, library=None):
     nin, nout = stagg_lengths(op, n)
     A = np.zeros((nout, nout))
     B = np.zeros((nout, nin))
-    ierr = lib().orc_stagg_np_dense(op, n, dz, flags, _p(A), _p(B))
+    ierr = (library or lib()).orc_stagg_np_dense(op, n, dz, flags, _p(A), _p(B))
     if ierr:
         raise OracleError(ierr)
     return A, B

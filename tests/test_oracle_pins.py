@@ -288,3 +288,24 @@ def test_error_codes():
     assert e.value.ierr == 324
     assert np.all(O.apply1d(O.CD06_D1, np.ones(1), 1.0) == 0)
     assert np.all(O.apply1d(O.CF90, np.full(1, 3.0), 1.0) == 3.0)
+
+
+@pytest.mark.parametrize("kind,op", [(1, O.CD06_D1), (2, O.STG_D1_E2C), (3, O.CD10_D1)])
+@pytest.mark.parametrize("n", [16, 24])
+def test_modified_wavenumbers_are_the_operators_eigenvalues(kind, op, n):
+    """The modified wavenumber of each kind (cd06, cd06stagg, cd10; the
+    Get_ModK callbacks of PoissonPeriodic.F90:188-204) is the eigenvalue of
+    the discrete periodic first-derivative operator on the wavenumbers the
+    Poisson solver uses: D exp(i k x_in) = i k' exp(i k x_out).  For the
+    staggered operator the input sits on edges and the output on cells."""
+    dx = 0.37
+    kp = O.modk(kind, n, dx)
+    k = O.modk(0, n, dx)
+    xin = np.arange(n + (1 if op == O.STG_D1_E2C else 0)) * dx
+    xout = np.arange(n) * dx + (0.5 * dx if op == O.STG_D1_E2C else 0.0)
+    for m in range(n):
+        re = O.apply(op, np.cos(k[m] * xin).reshape(1, 1, -1), 3, dx, True, n=n).ravel()[:n]
+        im = O.apply(op, np.sin(k[m] * xin).reshape(1, 1, -1), 3, dx, True, n=n).ravel()[:n]
+        got = re + 1j * im
+        expect = 1j * kp[m] * np.exp(1j * k[m] * xout)
+        assert np.abs(got - expect).max() <= 1e-12 * max(1.0, abs(kp[m])), (kind, m)

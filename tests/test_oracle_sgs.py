@@ -76,3 +76,43 @@ def test_dynamic_scale_invariance():
     lam3 = G.dynamic_procedure(sp, dx, dx, 2.0, u, v, w, sp.fft(u), sp.fft(v), sp.fft(w), S, D)
     assert np.any(lam1 > 0)
     assert np.allclose(lam1, lam3, rtol=1e-10, atol=0)
+
+
+def test_dynamic_coefficient_closed_form():
+    """Germano identity with the sharp test filter, in closed form.
+
+    u = A cos(2x) + B_z cos(5x), v = C cos(5x), w = 0 on a 16-point periodic
+    grid (k integer), uniform strain S = [[s, t, 0], [t, -s, 0], [0, 0, 0]]
+    and uniform kernel D.  With deltaRat = 2 the test filter keeps |k| < 8/3
+    (standardDynamicProcedure.F90 via spectral testFilter), so
+    F(u) = A cos 2x, F(v) = 0 and the planar means of the resolved stress are
+    <L11> = B^2/2, <L22> = C^2/2, <L12> = B C/2 (every other product mode,
+    |k| = 3, 4, 6, 7, 10, is removed).  The filtered strain is S itself and
+    its Smagorinsky kernel |S| = sqrt(2 S:S) = 2 sqrt(s^2 + t^2), so
+    M_ij = F(D S_ij) - deltaRat^2 |S| S_ij = (D - 4 |S|) S_ij = Dm S_ij, and
+    per plane
+        lambda = max(<L:M>, 0) / (2 <M:M>)
+               = Dm (s (B^2 - C^2)/2 + B C t) / (4 Dm^2 (s^2 + t^2)).
+    A dropped factor 2 on an off-diagonal term, a wrong deltaRat power or a
+    mean taken over the wrong axes changes the value."""
+    n, nz = 16, 3
+    dx = 2 * np.pi / n
+    sp = Spectral(n, n, nz, dx, dx, 0.1)
+    x = np.arange(n) * dx
+    Az, C, s, t, D = 0.9, 0.7, -0.5, -0.3, 1.7
+    Bz = np.array([1.2, 1.8, 2.4])
+    u = Az * np.cos(2 * x)[:, None, None] + Bz[None, None, :] * np.cos(5 * x)[:, None, None] * np.ones((1, n, 1))
+    v = C * np.cos(5 * x)[:, None, None] * np.ones((1, n, nz))
+    w = np.zeros((n, n, nz))
+    S = np.zeros((n, n, nz, 6))
+    S[..., 0], S[..., 3], S[..., 1] = s, -s, t
+    Dsgs = np.full((n, n, nz), D)
+    lam = G.dynamic_procedure(sp, dx, dx, 2.0, u, v, w, sp.fft(u), sp.fft(v), sp.fft(w), S, Dsgs)
+    Dm = D - 4 * 2 * np.sqrt(s * s + t * t)
+    num = Dm * (s * (Bz ** 2 - C ** 2) / 2 + Bz * C * t)
+    expect = np.maximum(num, 0) / (4 * Dm * Dm * (s * s + t * t))
+    assert np.all(expect > 0)
+    assert np.allclose(lam, expect, rtol=1e-12, atol=0)
+    # the opposite strain gives <L:M> < 0 in every plane: clipped to zero
+    lam = G.dynamic_procedure(sp, dx, dx, 2.0, u, v, w, sp.fft(u), sp.fft(v), sp.fft(w), -S, Dsgs)
+    assert np.all(lam == 0.0)
