@@ -289,10 +289,21 @@ def committed_traffic(kernel):
         return None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline():
-    """The oracle's FP64 full RHS (OpenMP C) on the host: evaluations at
-    256^3 repeated for at least 10 s (at most 8), the mean scaled by the
-    point count to a 512^3 evaluation rate."""
+    """The oracle's FP64 full RHS (OpenMP C, precomputed FFT twiddles) on the
+    host: one warm-up evaluation at 256^3, then the median of 10 timed ones,
+    scaled by the point count to a 512^3 evaluation rate."""
     import numpy as np
 
     import oracle as O
@@ -300,18 +311,18 @@ def cpu_baseline():
     n = 256
     rng = np.random.default_rng(3)
     u, v, w = (np.asfortranarray(rng.standard_normal((n, n, n))) for _ in range(3))
-    reps, t0 = 0, time.perf_counter()
-    while True:
+    O.full_rhs(u, v, w)
+    times = []
+    for _ in range(10):
+        t0 = time.perf_counter()
         O.full_rhs(u, v, w)
-        reps += 1
-        t = time.perf_counter() - t0
-        if t >= 10.0 or reps >= 8:
-            break
-    per = t / reps
+        times.append(time.perf_counter() - t0)
+    per = float(np.median(times))
     return {"value": round(1.0 / (per * 8.0), 5), "unit": "RHS evals/s (512^3-equivalent)",
             "cores": O.num_threads(), "kind": "port",
-            "sample": "%d FP64 full-RHS evaluations of the C oracle at 256^3 (%.1f s in all), rate scaled by "
-                      "1/8 to 512^3 points" % (reps, t)}
+            "sample": "median of 10 FP64 full-RHS evaluations of the C oracle at 256^3 after one warm-up "
+                      "(%.2f s each, %.1f s in all), rate scaled by 1/8 to 512^3 points; host CPU: %s"
+                      % (per, sum(times), cpu_model())}
 
 
 if __name__ == "__main__":

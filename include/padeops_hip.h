@@ -168,8 +168,9 @@ int pdo_op_dims(pdo_op h, int kind, int *n_sys, int *n_in);
  * (P == 1 with m == n means the general single-pencil kernel). */
 int pdo_op_partition(pdo_op h, int kind, int bc1, int bcn, int *m, int *P);
 /* Copy of the partitioned-solve tables of a variant (host memory, FP64), for
- * checking them on the CPU: meta = {m, P, K, nbl, nbr}; fac [4][m][5],
- * spk [4][m][4], G [P][2K][2KP], sp [2][4][8].  general != 0 asks for the
+ * checking them on the CPU: meta[6] = {m, P, K, nbl, nbr, xr} (the last
+ * segment holds m + xr rows, the others m); fac [4][m+xr][5],
+ * spk [4][m+xr][4], G [P][2K][2KP], sp [2][4][8].  general != 0 asks for the
  * single-segment tables.  Pass NULL arrays to query meta only. */
 int pdo_op_tables(pdo_op h, int kind, int bc1, int bcn, int general, int *meta, double *fac, double *spk,
                   double *G, double *sp);

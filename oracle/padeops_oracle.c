@@ -785,10 +785,21 @@ int orc_dense(int op, int n, double dx, int periodic, int bc1, int bcn, double *
 /* FFT (unnormalized, sign -1 forward) and the periodic Poisson solve          */
 /* ------------------------------------------------------------------------ */
 
-static void fft_line(double complex *x, int n, int sign, double complex *tmp)
+/* tw[k] = exp(sign 2 pi i k / n), k < n: the twiddle table of fft_line */
+static double complex *twiddles(int n, int sign)
+{
+    double complex *tw = malloc(sizeof(double complex) * n);
+    for (int k = 0; k < n; k++) {
+        const double ang = sign * 2.0 * M_PI * (double)k / n;
+        tw[k] = cos(ang) + I * sin(ang);
+    }
+    return tw;
+}
+
+static void fft_line(double complex *x, int n, const double complex *tw, double complex *tmp)
 {
     if ((n & (n - 1)) == 0) {
-        /* iterative radix-2 Cooley-Tukey */
+        /* iterative radix-2 Cooley-Tukey; stage `len` uses every (n/len)-th twiddle */
         for (int i = 1, j = 0; i < n; i++) {
             int bit = n >> 1;
             for (; j & bit; bit >>= 1) j ^= bit;
@@ -796,10 +807,10 @@ static void fft_line(double complex *x, int n, int sign, double complex *tmp)
             if (i < j) { double complex t = x[i]; x[i] = x[j]; x[j] = t; }
         }
         for (int len = 2; len <= n; len <<= 1) {
-            const double ang = sign * 2.0 * M_PI / len;
+            const int step = n / len;
             for (int i = 0; i < n; i += len)
                 for (int k = 0; k < len / 2; k++) {
-                    double complex w = cos(ang * k) + I * sin(ang * k);
+                    const double complex w = tw[k * step];
                     double complex u = x[i + k], v = x[i + k + len / 2] * w;
                     x[i + k] = u + v;
                     x[i + k + len / 2] = u - v;
@@ -808,10 +819,7 @@ static void fft_line(double complex *x, int n, int sign, double complex *tmp)
     } else {
         for (int k = 0; k < n; k++) {
             double complex s = 0;
-            for (int j = 0; j < n; j++) {
-                double ang = sign * 2.0 * M_PI * (double)((long)j * k % n) / n;
-                s += x[j] * (cos(ang) + I * sin(ang));
-            }
+            for (int j = 0; j < n; j++) s += x[j] * tw[(long)j * k % n];
             tmp[k] = s;
         }
         memcpy(x, tmp, sizeof(double complex) * n);
@@ -827,6 +835,7 @@ int orc_fft_c2c_dir(int dir, int n1, int n2, int n3, double *z, int sign)
     int oa = (dir == 1) ? 2 : 1, ob = (dir == 3) ? 2 : 3;
     long npen = (long)d[oa] * d[ob];
     const int n = d[dir];
+    double complex *tw = twiddles(n, sign);
 #pragma omp parallel
     {
         double complex *line = malloc(sizeof(double complex) * n), *tmp = malloc(sizeof(double complex) * n);
@@ -834,11 +843,12 @@ int orc_fft_c2c_dir(int dir, int n1, int n2, int n3, double *z, int sign)
         for (long p = 0; p < npen; p++) {
             long base = (p % d[oa]) * s[oa] + (p / d[oa]) * s[ob];
             for (int i = 0; i < n; i++) line[i] = a[base + i * s[dir]];
-            fft_line(line, n, sign, tmp);
+            fft_line(line, n, tw, tmp);
             for (int i = 0; i < n; i++) a[base + i * s[dir]] = line[i];
         }
         free(line); free(tmp);
     }
+    free(tw);
     return 0;
 }
 

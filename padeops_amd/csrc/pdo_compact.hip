@@ -81,11 +81,12 @@ __device__ __forceinline__ T stencil_row(const TW (&win)[W], int j, const T *c)
 }
 
 // right closure row against input columns [n_in - WBMAX, n_in): window index
-// MAXM + R - WBMAX + q + DN with DN = n_in - n_sys
-template <int FAM, int DN, typename T, typename TW, int W>
+// MSEG + R - WBMAX + q + DN with DN = n_in - n_sys (MSEG: rows of the last
+// segment)
+template <int FAM, int DN, int MSEG, typename T, typename TW, int W>
 __device__ __forceinline__ T right_closure(const TW (&win)[W], const double *sp)
 {
-    constexpr int base = MAXM + Fam<FAM>::R - WBMAX + DN;
+    constexpr int base = MSEG + Fam<FAM>::R - WBMAX + DN;
     static_assert(base >= 0 && base + WBMAX <= W, "closure window");
     T acc = 0;
 #pragma unroll
@@ -99,26 +100,42 @@ __device__ __forceinline__ T right_closure(const TW (&win)[W], const double *sp)
 // read one input point more (E2C: n_in = n_sys + 1) or less (C2E: n_in =
 // n_sys - 1) than they solve; the uniform branch on the difference keeps
 // every window index a compile-time constant.
-template <int FAM, typename T, typename TW, int W>
-__device__ __forceinline__ T rhs_row(const TW (&win)[W], int j, const T *c, const ApplyArgs &A, int ty, int m)
+template <int FAM, int MSEG, typename T, typename TW, int W>
+__device__ __forceinline__ T right_closure_dn(const TW (&win)[W], const double *sp, int dn)
+{
+    if (dn == 0) return right_closure<FAM, 0, MSEG, T>(win, sp);
+    if (dn > 0) return right_closure<FAM, 1, MSEG, T>(win, sp);
+    return right_closure<FAM, -1, MSEG, T>(win, sp);
+}
+
+// (closure rows are at most NBMAX per end, so rows farther in are pruned at
+// compile time; `ext`: this is the last segment of a system of 32 P + 1 rows,
+// which holds row MAXM too)
+template <int FAM, typename T, int XR = 0, typename TW, int W>
+__device__ __forceinline__ T rhs_row(const TW (&win)[W], int j, const T *c, const ApplyArgs &A, int ty, int m,
+                                     bool ext = false)
 {
     using F = Fam<FAM>;
     static_assert(MAXM + F::R - WBMAX >= 1, "closure window");
     if (!A.periodic) {
-        if ((ty == 0 || ty == 3) && j < A.nbl) {
+        if ((ty == 0 || ty == 3) && j < NBMAX && j < A.nbl) {
             const double *sp = A.tab_d + A.o_sp + j * WBMAX;
             T acc = 0;
 #pragma unroll
             for (int q = 0; q < WBMAX; ++q) acc = fma((T)sp[q], (T)win[q + F::R], acc);
             return acc;
         }
-        if ((ty == 2 || ty == 3) && j >= m - A.nbr) {
-            const double *sp = A.tab_d + A.o_sp + (NBMAX + j - (m - A.nbr)) * WBMAX;
-            const int dn = A.n_in - A.n_sys;
-            if (dn == 0) return right_closure<FAM, 0, T>(win, sp);
-            if (dn > 0) return right_closure<FAM, 1, T>(win, sp);
-            return right_closure<FAM, -1, T>(win, sp);
+        const int dn = A.n_in - A.n_sys;
+        if constexpr (XR > 0) {
+            if ((ty == 2 || ty == 3) && ext) {
+                if (j >= MAXM + 1 - NBMAX && j >= MAXM + 1 - A.nbr)
+                    return right_closure_dn<FAM, MAXM + 1, T>(
+                        win, A.tab_d + A.o_sp + (NBMAX + j - (MAXM + 1 - A.nbr)) * WBMAX, dn);
+                return stencil_row<FAM, T>(win, j, c);
+            }
         }
+        if ((ty == 2 || ty == 3) && j >= m - NBMAX && j >= m - A.nbr)
+            return right_closure_dn<FAM, MAXM, T>(win, A.tab_d + A.o_sp + (NBMAX + j - (m - A.nbr)) * WBMAX, dn);
     }
     return stencil_row<FAM, T>(win, j, c);
 }
@@ -151,6 +168,64 @@ __device__ __forceinline__ void local_solve(T (&d)[MM], PT fac, int m)
             d[j] = v;
         }
         PDO_SCHED_FENCE(j, 8);
+    }
+}
+
+// the same for segments of MAXM rows, or MAXM + 1 when `ext` (MM = MAXM + 1)
+template <int K, int MM, typename T, typename PT>
+__device__ __forceinline__ void local_solve_x(T (&d)[MM], PT fac, bool ext)
+{
+#pragma unroll
+    for (int j = 1; j < MM; ++j) {
+        if (j < MAXM || ext) {
+            T v = d[j] - fac[j * 5 + 0] * d[j - 1];
+            if (K == 2 && j >= 2) v -= fac[j * 5 + 1] * d[j - 2];
+            d[j] = v;
+        }
+        PDO_SCHED_FENCE(j, 8);
+    }
+#pragma unroll
+    for (int j = MM - 1; j >= 0; --j) {
+        if (j < MAXM || ext) {
+            T v = d[j] * fac[j * 5 + 2];
+            if (j + 1 < MM && (j + 1 < MAXM || ext)) v -= fac[j * 5 + 3] * d[j + 1];
+            if (K == 2 && j + 2 < MM && (j + 2 < MAXM || ext)) v -= fac[j * 5 + 4] * d[j + 2];
+            d[j] = v;
+        }
+        PDO_SCHED_FENCE(j, 8);
+    }
+}
+
+template <int K, int MM, typename T, typename PT>
+__device__ __forceinline__ void apply_spikes_x(T (&d)[MM], PT spk, const T (&yl)[2], const T (&yr)[2], bool ext)
+{
+#pragma unroll
+    for (int j = 0; j < MM; ++j) {
+        if (j < MAXM || ext) {
+            T v = d[j];
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                v = fma(-spk[j * 4 + q], yl[q], v);
+                v = fma(-spk[j * 4 + 2 + q], yr[q], v);
+            }
+            d[j] = v;
+        }
+        PDO_SCHED_FENCE(j, 8);
+    }
+}
+
+// publish the first and last K rows of a segment (the last K are rows
+// MAXM - K .. MAXM - 1, or one further when ext) at slots s*2K + r of an
+// exchange buffer with NPW values per slot
+template <int K, int NPW, int MM, typename T>
+__device__ __forceinline__ void publish_ends(T *rb, const T (&d)[MM], int s, int pl, bool ext)
+{
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        rb[(s * 2 * K + q) * NPW + pl] = d[q];
+        T v = d[MAXM - K + q];
+        if constexpr (MM > MAXM) v = ext ? d[MAXM + 1 - K + q] : v;
+        rb[(s * 2 * K + K + q) * NPW + pl] = v;
     }
 }
 
@@ -295,55 +370,53 @@ __device__ __forceinline__ void reduce_seg(const typename Mode<MODE>::TC *red, c
 
 // the solve of the strided kernel: RHS from this lane's window, local solve,
 // interface exchange through `red` (LDS), spikes, and the FP64-residual
-// refinement; leaves this lane's 32 output rows in d
-template <int FAM, int MODE, int NPW, bool PER>
-__device__ __forceinline__ void strided_compute(const ApplyArgs &A, const typename Mode<MODE>::TS (&win)[MAXM + 2 * Fam<FAM>::R],
+// refinement; leaves this lane's output rows in d.  XR = 1: the last segment
+// of a non-periodic system of 32 P + 1 rows holds MAXM + 1 rows (`ext`);
+// the others MAXM.
+template <int FAM, int MODE, int NPW, bool PER, int XR>
+__device__ __forceinline__ void strided_compute(const ApplyArgs &A,
+                                                const typename Mode<MODE>::TS (&win)[MAXM + XR + 2 * Fam<FAM>::R],
                                                 typename Mode<MODE>::TC *red, int s, int pl, int ty,
                                                 const typename Mode<MODE>::TC *fac, const typename Mode<MODE>::TC *spk,
-                                                typename Mode<MODE>::TC (&d)[MAXM])
+                                                typename Mode<MODE>::TC (&d)[MAXM + XR], bool ext)
 {
     using TC = typename Mode<MODE>::TC;
     using F = Fam<FAM>;
-    constexpr int K = F::K, MM = MAXM;
+    constexpr int K = F::K, MM = MAXM + XR;
     constexpr bool REFINE = Mode<MODE>::REFINE;
     using TR = typename std::conditional<REFINE, double, TC>::type;  // RHS arithmetic
-    const int m = MM, P = A.P;
+    const int m = MAXM, P = A.P;
+    // rows of this segment
+    const int ms = MAXM + (ext ? 1 : 0);
     TR cr[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) cr[q] = (TR)A.c[q];
 #pragma unroll
     for (int j = 0; j < MM; ++j) {
         if (PER) d[j] = (TC)stencil_row<FAM, TR>(win, j, cr);
-        else d[j] = (TC)rhs_row<FAM, TR>(win, j, cr, A, ty, m);
+        else if (j < MAXM || ext) d[j] = (TC)rhs_row<FAM, TR, XR>(win, j, cr, A, ty, m, ext);
+        else d[j] = TC(0);
     }
 
     const TC *Gs = tab<TC>(A) + A.o_G;
     TC yl[2], yr[2];
 
-    local_solve<K, MM, true>(d, fac, m);
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-        red[(s * 2 * K + q) * NPW + pl] = d[q];
-        red[(s * 2 * K + K + q) * NPW + pl] = d[m - K + q];
-    }
+    local_solve_x<K, MM>(d, fac, ext);
+    publish_ends<K, NPW>(red, d, s, pl, ext);
     __syncthreads();
     // coupling reach of the first solve (and of the only one, unrefined)
     const int D1 = (sizeof(TC) == 8) ? A.D64 : (REFINE ? A.D32R : A.D32);
     reduce_seg<FAM, MODE, NPW, PER>(red, A, s, pl, yl, yr, Gs, D1);
-    apply_spikes<K, MM, true>(d, spk, yl, yr, m);
+    apply_spikes_x<K, MM>(d, spk, yl, yr, ext);
 
     if constexpr (REFINE) {
         // residual r = B f - A y in FP64.  The neighbour rows it needs (rows
-        // -K..-1 and m..m+K-1) are taken from the values the neighbouring
+        // -K..-1 and ms..ms+K-1) are taken from the values the neighbouring
         // segments actually hold, published through LDS, so that the residual
         // belongs to the stored solution and the correction removes its error.
         TC *red2 = red + P * 2 * K * NPW;
         TC *red3 = red2 + P * 2 * K * NPW;
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            red3[(s * 2 * K + q) * NPW + pl] = d[q];
-            red3[(s * 2 * K + K + q) * NPW + pl] = d[m - K + q];
-        }
+        publish_ends<K, NPW>(red3, d, s, pl, ext);
         __syncthreads();
         {
             const int pv = s > 0 ? s - 1 : (PER ? P - 1 : -1);
@@ -354,32 +427,37 @@ __device__ __forceinline__ void strided_compute(const ApplyArgs &A, const typena
                 yr[q] = nx >= 0 ? red3[(nx * 2 * K + q) * NPW + pl] : TC(0);
             }
         }
-        // periodic rows all share the interior band (kernel arguments);
-        // non-periodic rows read theirs from the table
-        const double *band = A.tab_d + A.o_lhs + ty * m * 5;
+        // periodic rows all share the interior band (kernel arguments); so do
+        // the rows of a non-periodic system farther than NBMAX from its ends,
+        // the others read theirs from the table
+        const double *band = A.tab_d + A.o_lhs + ty * (m + A.xr) * 5;
         TC e[MM];
 #pragma unroll
         for (int j = 0; j < MM; ++j) {
-            double acc = PER ? stencil_row<FAM, double>(win, j, cr) : rhs_row<FAM, double>(win, j, cr, A, ty, m);
+            if (!PER && !(j < MAXM || ext)) { e[j] = TC(0); continue; }
+            double acc = PER ? stencil_row<FAM, double>(win, j, cr) : rhs_row<FAM, double, XR>(win, j, cr, A, ty, m, ext);
+            const bool tab_row = !PER && (((ty == 0 || ty == 3) && j < NBMAX) ||
+                                          ((ty == 2 || ty == 3) && j >= MAXM - NBMAX && j >= ms - NBMAX));
 #pragma unroll
             for (int o = -K; o <= K; ++o) {
                 const int jj = j + o;
-                const double yv = jj < 0 ? (double)yl[K + jj]
-                                         : (jj >= m ? (double)yr[jj - m] : (double)d[jj < 0 ? 0 : (jj >= MM ? MM - 1 : jj)]);
-                acc = fma(-(PER ? A.band[o + 2] : band[j * 5 + o + 2]), yv, acc);
+                TC y;
+                if (jj < 0) y = yl[K + jj];
+                else if (jj < MAXM) y = d[jj];
+                else if (XR == 0) y = yr[jj - MAXM < 1 ? jj - MAXM : 1];
+                else if (jj == MAXM) y = ext ? d[MM - 1] : yr[0];
+                else y = ext ? yr[jj - MAXM - 1] : yr[jj - MAXM < 1 ? jj - MAXM : 1];
+                const double yv = (double)y;
+                acc = fma(-(tab_row ? band[j * 5 + o + 2] : A.band[o + 2]), yv, acc);
             }
             e[j] = (TC)acc;
         }
-        local_solve<K, MM, true>(e, fac, m);
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            red2[(s * 2 * K + q) * NPW + pl] = e[q];
-            red2[(s * 2 * K + K + q) * NPW + pl] = e[m - K + q];
-        }
+        local_solve_x<K, MM>(e, fac, ext);
+        publish_ends<K, NPW>(red2, e, s, pl, ext);
         __syncthreads();
         TC el[2], er[2];
         reduce_seg<FAM, MODE, NPW, PER>(red2, A, s, pl, el, er, Gs, A.D32C);
-        apply_spikes<K, MM, true>(e, spk, el, er, m);
+        apply_spikes_x<K, MM>(e, spk, el, er, ext);
 #pragma unroll
         for (int j = 0; j < MM; ++j) d[j] += e[j];
     }
@@ -387,10 +465,10 @@ __device__ __forceinline__ void strided_compute(const ApplyArgs &A, const typena
 }
 
 // rows of this lane's segment to out (strided by se_out), optionally
-// out = aux +- d
-template <int MODE, bool EPI>
-__device__ __forceinline__ void strided_store(const ApplyArgs &A, const typename Mode<MODE>::TC (&d)[MAXM], long a,
-                                              int s)
+// out = aux +- d; row MAXM too when ext
+template <int MODE, bool EPI, int XR>
+__device__ __forceinline__ void strided_store(const ApplyArgs &A, const typename Mode<MODE>::TC (&d)[MAXM + XR],
+                                              long a, int s, bool ext)
 {
     using TS = typename Mode<MODE>::TS;
     using TC = typename Mode<MODE>::TC;
@@ -433,6 +511,10 @@ __device__ __forceinline__ void strided_store(const ApplyArgs &A, const typename
 #pragma unroll
         for (int j = 0; j < MM; ++j) o[po + (unsigned)(s * m + j) * so] = (TS)d[j];
     }
+    if constexpr (XR > 0) {
+        // the extra row of the last segment (no epilogue on this path)
+        if (ext) buf_st((TS)d[MAXM], ro, vo, (unsigned)MAXM * ss);
+    }
     if (A.n_out > A.n_sys && s == 0) {
         // periodic edge layouts repeat row 0 at row n_sys
         const unsigned off = (unsigned)A.n_sys * ss;
@@ -445,16 +527,17 @@ __device__ __forceinline__ void strided_store(const ApplyArgs &A, const typename
     }
 }
 
-template <int FAM, int MODE, int NPW, bool PER, bool EPI>
-__global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
+// LB: workgroup size bound, 1024 (up to 16 segments) or 512 (up to 8, which
+// leaves the compiler 256 vector registers per lane)
+template <int FAM, int MODE, int NPW, bool PER, bool EPI, int XR, int LB>
+__global__ void __launch_bounds__(LB) k_strided(ApplyArgs A)
 {
     using TS = typename Mode<MODE>::TS;
     using TC = typename Mode<MODE>::TC;
     using F = Fam<FAM>;
-    constexpr int K = F::K, R = F::R, MM = MAXM, W = MM + 2 * R;
-    constexpr bool REFINE = Mode<MODE>::REFINE;
+    constexpr int K = F::K, R = F::R, MM = MAXM, W = MM + XR + 2 * R;
     static_assert(NPW == 64 || PER, "two segments per wave need uniform (periodic) tables");
-    using TR = typename std::conditional<REFINE, double, TC>::type;  // RHS arithmetic
+    static_assert(XR == 0 || (!PER && !EPI), "extra rows: non-periodic systems without epilogue");
     const int m = MM, P = A.P;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -468,6 +551,7 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
     const long a = (long)bx * NPW + pl;
     const bool act = a < A.na;
     const int ty = PER ? 1 : (P == 1 ? 3 : (s == 0 ? 0 : (s == P - 1 ? 2 : 1)));
+    const bool ext = XR > 0 && s == P - 1;
 
     // byte offsets fit 32 bits (the host sends larger fields elsewhere)
     const __amdgpu_buffer_rsrc_t rs = field_rsrc(A.f);
@@ -503,12 +587,12 @@ __global__ void __launch_bounds__(1024) k_strided(ApplyArgs A)
 
     extern __shared__ unsigned char smem[];
     TC *red = (TC *)smem;
-    const TC *fac = tab<TC>(A) + A.o_fac + ty * m * 5;
-    const TC *spk = tab<TC>(A) + A.o_spk + ty * m * 4;
-    TC d[MM];
-    strided_compute<FAM, MODE, NPW, PER>(A, win, red, s, pl, ty, fac, spk, d);
+    const TC *fac = tab<TC>(A) + A.o_fac + ty * (m + A.xr) * 5;
+    const TC *spk = tab<TC>(A) + A.o_spk + ty * (m + A.xr) * 4;
+    TC d[MM + XR];
+    strided_compute<FAM, MODE, NPW, PER, XR>(A, win, red, s, pl, ty, fac, spk, d, ext);
     if (!act) return;
-    strided_store<MODE, EPI>(A, d, a, s);
+    strided_store<MODE, EPI, XR>(A, d, a, s, ext);
 }
 
 // exchange buffers of the refined x kernel: 3 (one per exchange) or 1
@@ -865,17 +949,22 @@ hipError_t launch_zero_words(void *p, long words, hipStream_t st)
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int FAM, int MODE, int NPW, bool PER, bool EPI>
+template <int FAM, int MODE, int NPW, bool PER, bool EPI, int XR = 0, int LB = 1024>
 static hipError_t launch_strided_k(const ApplyArgs &A, hipStream_t st)
 {
     using TC = typename Mode<MODE>::TC;
     constexpr int K = Fam<FAM>::K;
     dim3 grid((unsigned)((A.na + NPW - 1) / NPW), (unsigned)A.nb);
     dim3 block(NPW * A.P);
+    if ((int)block.x > LB || A.xr != XR) return hipErrorInvalidValue;
     size_t lds = (size_t)A.P * 2 * K * NPW * sizeof(TC) * (Mode<MODE>::REFINE ? 3 : 1);
-    hipLaunchKernelGGL((k_strided<FAM, MODE, NPW, PER, EPI>), grid, block, lds, st, A);
+    hipLaunchKernelGGL((k_strided<FAM, MODE, NPW, PER, EPI, XR, LB>), grid, block, lds, st, A);
     return hipGetLastError();
 }
+
+// systems of 32 P + 1 rows on the register path: the edge systems of the
+// staggered operators (tridiagonal families)
+template <int FAM> constexpr bool has_extra_row() { return Fam<FAM>::K == 1; }
 
 // epilogue operands (divergence accumulation, projection subtraction) are
 // only used with the periodic cd06 first derivative
@@ -892,7 +981,15 @@ static hipError_t launch_strided_t(const ApplyArgs &A, hipStream_t st)
 {
     if (!A.periodic) {
         if (A.epi) return hipErrorInvalidValue;
-        return launch_strided_k<FAM, MODE, 64, false, false>(A, st);
+        if (A.xr) {
+            if constexpr (has_extra_row<FAM>()) {
+                if (A.P <= 8) return launch_strided_k<FAM, MODE, 64, false, false, 1, 512>(A, st);
+                return launch_strided_k<FAM, MODE, 64, false, false, 1, 1024>(A, st);
+            }
+            return hipErrorInvalidValue;
+        }
+        if (A.P <= 8) return launch_strided_k<FAM, MODE, 64, false, false, 0, 512>(A, st);
+        return launch_strided_k<FAM, MODE, 64, false, false, 0, 1024>(A, st);
     }
     if (A.epi) {
         if constexpr (has_epilogue<FAM>()) {

@@ -26,6 +26,7 @@ struct ApplyArgs {
     long sa_out, sb_out, se_out;
     int n_sys, n_in, n_read, n_out;
     int m, P;
+    int xr;            // rows the last segment holds beyond m (0 or 1)
     int lgn;           // log2(n_in) (x kernel; n_in is a power of two there)
     int D32, D64;      // circulant coupling offsets kept by FP32 / FP64 solves
     int D32R, D32C;    // ... by the refined mode's first solve / its correction

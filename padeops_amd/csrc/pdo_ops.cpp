@@ -49,11 +49,6 @@ int TabSet::upload()
     return 0;
 }
 
-Variant::~Variant()
-{
-    if (scratch) (void)hipFree(scratch);
-}
-
 static int build_desc(const OpObject &o, int kind, int bc1, int bcn, SystemDesc &s)
 {
     if (o.scheme == SCHEME_CF90 && kind == KIND_FILTER + 1000) {
@@ -83,6 +78,15 @@ static int build_desc(const OpObject &o, int kind, int bc1, int bcn, SystemDesc 
     return PDO_ERR_ARG;
 }
 
+Variant *OpObject::find_variant(int kind, int b1, int bn)
+{
+    if (periodic) b1 = bn = 0;
+    const int key = kind * 100 + (b1 + 1) * 10 + (bn + 1);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = variants.find(key);
+    return it == variants.end() ? nullptr : it->second.get();
+}
+
 int OpObject::variant(int kind, int b1, int bn, Variant **out)
 {
     if (periodic) b1 = bn = 0;  // periodic systems ignore the boundary flags
@@ -99,6 +103,12 @@ int OpObject::variant(int kind, int b1, int bn, Variant **out)
     st = build_spike(v->sys, v->fast.tab);
     if (st) { set_error("failed to build the partitioned-solve tables"); return PDO_ERR_ARG; }
     if ((st = v->fast.upload())) return st;
+    // the single-segment tables of the general kernel too (a few KB), so that
+    // every shape the variant can take is ready once the variant exists
+    if (!v->fast.tab.general) {
+        if (build_spike(v->sys, v->gen.tab, true)) { set_error("failed to build tables"); return PDO_ERR_ARG; }
+        if ((st = v->gen.upload())) return st;
+    }
     *out = v.get();
     variants[key] = std::move(v);
     return 0;
@@ -119,7 +129,7 @@ int prec_mode(int prec)
 // there is nothing to do (na or nb zero).
 static int prepare(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, const void *f, void *out, long na,
                    long nb, int epi, const void *aux, int force_shape, ApplyArgs &A, int &shape, int &mode,
-                   int &family, double *&scratch, bool *empty)
+                   int &family, size_t &scratch_elems, bool *empty, bool noalloc = false)
 {
     *empty = false;
     if (!op || !f || !out || na < 0 || nb < 0 || dir < 1 || dir > 3) {
@@ -131,8 +141,17 @@ static int prepare(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, 
     if (epi && !aux) { set_error("epilogue without operand"); return PDO_ERR_ARG; }
     if (na == 0 || nb == 0) { *empty = true; return 0; }
     Variant *v = nullptr;
-    int st = op->variant(kind, bc1, bcn, &v);
-    if (st) return st;
+    int st = 0;
+    if (noalloc) {
+        // building a variant uploads its tables (device allocations)
+        if (!(v = op->find_variant(kind, bc1, bcn))) {
+            set_error("operator variant not built yet: one call of the allocating entry point with this handle, "
+                      "kind and boundary flags builds it");
+            return PDO_ERR_ARG;
+        }
+    } else if ((st = op->variant(kind, bc1, bcn, &v))) {
+        return st;
+    }
     const SystemDesc &s = v->sys;
     long A_na, A_nb;
     if (dir == 1) { A_na = na; A_nb = nb; shape = SHAPE_X; }
@@ -152,13 +171,14 @@ static int prepare(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, 
     const size_t word = (mode == MODE_F64) ? 8 : 4;
     const size_t nmax = (size_t)std::max(std::max(s.n_in, s.n_out), s.n_sys + 1);
     if (shape == SHAPE_STRIDED && (size_t)na * (size_t)nb * nmax * word >= ((size_t)1 << 32)) shape = SHAPE_GENERAL;
+    // the register path's non-periodic kernels take no epilogue operand, and
+    // an extra row in the last segment only in the tridiagonal families
+    if (shape == SHAPE_STRIDED && !s.periodic && (epi || (v->fast.tab.xr && family_shape(s.family).K != 1)))
+        shape = SHAPE_GENERAL;
     TabSet *ts = &v->fast;
     if (shape == SHAPE_GENERAL && !v->fast.tab.general) {
-        std::lock_guard<std::mutex> lk(op->mu);
-        if (!v->gen.ready) {
-            if (build_spike(s, v->gen.tab, true)) { set_error("failed to build tables"); return PDO_ERR_ARG; }
-            if ((st = v->gen.upload())) return st;
-        }
+        // built with the variant
+        if (!v->gen.ready) { set_error("single-segment tables missing"); return PDO_ERR_ARG; }
         ts = &v->gen;
     }
     const SpikeTables &t = ts->tab;
@@ -168,7 +188,7 @@ static int prepare(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, 
     A.periodic = s.periodic ? 1 : 0;
     A.n_sys = s.n_sys; A.n_in = s.n_in; A.n_out = s.n_out;
     A.n_read = (s.family == FAM_STG_E2C_A || s.family == FAM_STG_E2C_S) ? s.n_sys + 1 : s.n_sys;
-    A.m = t.m; A.P = t.P; A.nbl = t.nbl; A.nbr = t.nbr; A.wb = t.wb;
+    A.m = t.m; A.P = t.P; A.xr = t.xr; A.nbl = t.nbl; A.nbr = t.nbr; A.wb = t.wb;
     A.D32 = t.D32; A.D64 = t.D64; A.D32R = t.D32R; A.D32C = t.D32C;
     A.lgn = 0;
     while ((1 << A.lgn) < s.n_in) A.lgn++;
@@ -190,20 +210,7 @@ static int prepare(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, 
         A.sa_in = 1; A.sb_in = 0; A.se_in = na * nb;
         A.sa_out = 1; A.sb_out = 0; A.se_out = na * nb;
     }
-    scratch = nullptr;
-    if (shape == SHAPE_GENERAL) {
-        const size_t need = (size_t)s.n_sys * (size_t)(A.na * A.nb);
-        std::lock_guard<std::mutex> lk(op->mu);
-        if (v->scratch_elems < need) {
-            if (v->scratch) (void)hipFree(v->scratch);
-            v->scratch = nullptr;
-            v->scratch_elems = 0;
-            hipError_t e = hipMalloc(&v->scratch, need * sizeof(double));
-            if (e != hipSuccess) return hip_fail(e, "hipMalloc(scratch)");
-            v->scratch_elems = need;
-        }
-        scratch = v->scratch;
-    }
+    scratch_elems = (shape == SHAPE_GENERAL) ? (size_t)s.n_sys * (size_t)(A.na * A.nb) : 0;
     return 0;
 }
 
@@ -212,16 +219,69 @@ int apply(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, const voi
 {
     ApplyArgs A;
     int shape = 0, mode = 0, family = 0;
-    double *scratch = nullptr;
+    size_t scratch_elems = 0;
     bool empty = false;
     int st = prepare(op, kind, bc1, bcn, dir, prec, f, out, na, nb, epi, aux, force_shape, A, shape, mode, family,
-                     scratch, &empty);
+                     scratch_elems, &empty);
     if (st || empty) return st;
-    hipError_t e = launch_compact(family, shape, mode, A, scratch, stream);
+    // the general kernel's work area is stream-ordered memory of this call
+    // alone: no host synchronisation, and no sharing between launches that
+    // run concurrently on different streams
+    double *scratch = nullptr;
+    hipError_t e = hipSuccess;
+    if (scratch_elems) {
+        e = hipMallocAsync((void **)&scratch, scratch_elems * sizeof(double), stream);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(general-kernel work area)");
+    }
+    e = launch_compact(family, shape, mode, A, scratch, stream);
+    if (scratch) {
+        const hipError_t ef = hipFreeAsync(scratch, stream);
+        if (e == hipSuccess) e = ef;
+    }
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return 0;
 }
 
+int apply_noalloc(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, const void *f, void *out, long na,
+                  long nb, hipStream_t stream, double *work, size_t work_elems)
+{
+    if (!op || !f || !out || na < 0 || nb < 0 || dir < 1 || dir > 3 || prec_mode(prec) < 0) {
+        set_error("invalid argument");
+        return PDO_ERR_ARG;
+    }
+    if (op->scheme == SCHEME_CD10 || op->scheme == SCHEME_CF90) {
+        // cd10.F90:2042-2058, cf90.F90:1036-1052: GracefulExit(..., 324)
+        if (bc1 < -1 || bc1 > 1 || bcn < -1 || bcn > 1) {
+            set_error("Incorrect boundary specification for bc1/bcn (should be 0, 1 or -1)");
+            return 324;
+        }
+    } else {
+        bc1 = bcn = 0;  // cd06 and cd06stagg objects take their ends from init
+    }
+    if (op->n == 1) {
+        // one-point objects: derivatives are zero, the filter is the identity
+        const size_t w = (prec == PDO_F64) ? 8 : 4;
+        hipError_t e = (kind == KIND_FILTER)
+                           ? hipMemcpyAsync(out, f, (size_t)(na * nb) * w, hipMemcpyDeviceToDevice, stream)
+                           : launch_zero_words(out, na * nb * (long)(w / 4), stream);
+        return e == hipSuccess ? 0 : hip_fail(e, "one-point operator");
+    }
+    ApplyArgs A;
+    int shape = 0, mode = 0, family = 0;
+    size_t scratch_elems = 0;
+    bool empty = false;
+    int st = prepare(op, kind, bc1, bcn, dir, prec, f, out, na, nb, 0, nullptr, -1, A, shape, mode, family,
+                     scratch_elems, &empty, true);
+    if (st || empty) return st;
+    if (scratch_elems > work_elems) {
+        set_error("input too large for the fixed work area of the allocation-free entry point (" +
+                  std::to_string(scratch_elems) + " > " + std::to_string(work_elems) + " FP64 words)");
+        return PDO_ERR_ARG;
+    }
+    const hipError_t e = launch_compact(family, shape, mode, A, scratch_elems ? work : nullptr, stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return 0;
+}
 
 }  // namespace pdo
 
@@ -506,7 +566,7 @@ int pdo_op_tables(pdo_op h, int kind, int bc1, int bcn, int general, int *meta, 
     if (st) return st;
     SpikeTables t;
     if (build_spike(s, t, general != 0)) return PDO_ERR_ARG;
-    meta[0] = t.m; meta[1] = t.P; meta[2] = t.K; meta[3] = t.nbl; meta[4] = t.nbr;
+    meta[0] = t.m; meta[1] = t.P; meta[2] = t.K; meta[3] = t.nbl; meta[4] = t.nbr; meta[5] = t.xr;
     if (fac) std::memcpy(fac, t.fac.data(), t.fac.size() * sizeof(double));
     if (spk) std::memcpy(spk, t.spk.data(), t.spk.size() * sizeof(double));
     if (G) std::memcpy(G, t.G.data(), t.G.size() * sizeof(double));

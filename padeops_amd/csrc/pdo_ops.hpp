@@ -31,9 +31,6 @@ struct Variant {
     SystemDesc sys;
     TabSet fast;   // partitioned tables (general == true when n does not fit)
     TabSet gen;    // single-segment tables of the general kernel, built on demand
-    double *scratch = nullptr;  // general kernel work area
-    size_t scratch_elems = 0;
-    ~Variant();
 };
 
 struct OpObject {
@@ -47,6 +44,8 @@ struct OpObject {
     std::mutex mu;
     // build (or fetch) a variant; returns status (0 ok)
     int variant(int kind, int bc1, int bcn, Variant **out);
+    // an already built variant, or nullptr (builds nothing)
+    Variant *find_variant(int kind, int bc1, int bcn);
 };
 
 // Apply one variant along dir (1 x, 2 y, 3 z) of a field whose pencil count
@@ -54,6 +53,12 @@ struct OpObject {
 // epi 2 out = aux - y.
 int apply(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, const void *f, void *out, long na, long nb,
           hipStream_t stream, int epi = 0, const void *aux = nullptr, int force_shape = -1);
+
+// apply() without any device allocation, for pdo_op_apply_noalloc: the
+// variant must already be built, and the general kernel's work area is
+// `work` (work_elems FP64 words); larger inputs are rejected
+int apply_noalloc(OpObject *op, int kind, int bc1, int bcn, int dir, int prec, const void *f, void *out, long na,
+                  long nb, hipStream_t stream, double *work, size_t work_elems);
 
 int prec_mode(int prec);
 

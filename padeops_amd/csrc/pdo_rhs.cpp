@@ -36,7 +36,10 @@ struct RhsObject {
     double L = 0;
     int prec = 0;
     size_t word = 8;
-    OpObject d1, fil;
+    // d1g serves the gradient passes on the side stream, d1 those on the
+    // caller's stream: one object per stream, so that no variant state (the
+    // general kernel's work area) is shared by concurrent launches
+    OpObject d1, d1g, fil;
     PoissonObject poisson;
     void *ut[3] = {nullptr, nullptr, nullptr};
     void *tmp = nullptr, *div = nullptr;
@@ -76,7 +79,9 @@ int pdo_rhs_init(pdo_rhs *h, int n, double L, int prec)
     auto *o = new pdo_rhs_s();
     o->n = n; o->L = L; o->prec = prec; o->word = (prec == PDO_F64) ? 8 : 4;
     const double dx = L / n;
-    o->d1.scheme = SCHEME_CD06; o->d1.n = n; o->d1.dx = dx; o->d1.periodic = true;
+    for (OpObject *d : {&o->d1, &o->d1g}) {
+        d->scheme = SCHEME_CD06; d->n = n; d->dx = dx; d->periodic = true;
+    }
     o->fil.scheme = SCHEME_CF90; o->fil.n = n; o->fil.dx = 1.0; o->fil.periodic = true;
     int st = o->poisson.init(n, n, n, dx, dx, dx, PDO_MODK_CD06, PDO_MODK_CD06, PDO_MODK_CD06, 1);
     if (st) { delete o; return st; }
@@ -124,7 +129,7 @@ int pdo_rhs_eval(pdo_rhs h, const void *u, const void *v, const void *w, void *g
         if (ee != hipSuccess) return hip_fail(ee, "stream fork");
         for (int i = i0; i < i1; i++)
             for (int j = 0; j < 3; j++) {
-                const int rr = op_on(PDO_RHS_SIDE_STREAM ? h->side : st, &h->d1, KIND_D1, j + 1, ui[i],
+                const int rr = op_on(PDO_RHS_SIDE_STREAM ? h->side : st, &h->d1g, KIND_D1, j + 1, ui[i],
                                      h->field(grad, 3 * i + j), 0, nullptr);
                 if (rr) return rr;
             }

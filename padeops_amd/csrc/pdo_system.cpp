@@ -624,13 +624,17 @@ void dense(const SystemDesc &s, double *A, double *B)
 // ---------------------------------------------------------------------------
 // Partitioned solve tables
 // ---------------------------------------------------------------------------
-bool choose_partition(int n_sys, int K, int nb, int &m, int &P)
+bool choose_partition(int n_sys, int K, int nb, bool periodic, int &m, int &P, int &xr)
 {
-    // the register-resident kernels hold exactly MAXM rows per lane
+    // the register-resident kernels hold MAXM rows per lane; a non-periodic
+    // system of 32 P + 1 rows (the edge systems of a staggered grid of 32 P
+    // cells) gives its last segment one row more
     (void)K; (void)nb;
     m = MAXM;
     P = n_sys / MAXM;
-    return n_sys % MAXM == 0 && P >= 1 && P <= PMAX;
+    xr = n_sys % MAXM;
+    if (xr > 1 || (xr == 1 && periodic)) return false;
+    return P >= 1 && P <= PMAX;
 }
 
 // dense Gauss-Jordan inverse with partial pivoting (small matrices)
@@ -673,14 +677,18 @@ int build_spike(const SystemDesc &s, SpikeTables &t, bool force_general)
     }
     if (nbl > NBMAX || nbr > NBMAX || wb > WBMAX) return -1;
     if (!s.special.empty() && s.n_in < WBMAX) return -1;
-    int m, P;
-    bool fast = !force_general && choose_partition(n, K, std::max(nbl, nbr), m, P);
-    if (!fast) { m = n; P = 1; }
+    int m, P, xr;
+    bool fast = !force_general && choose_partition(n, K, std::max(nbl, nbr), s.periodic, m, P, xr);
+    if (!fast) { m = n; P = 1; xr = 0; }
+    // rows of a segment of type ty (the last, or the only, segment holds xr
+    // more), and the row stride of the per-type tables
+    const int mt = m + xr;
+    auto rows_of = [&](int ty) { return (ty == 2 || ty == 3) ? m + xr : m; };
     t = SpikeTables();
-    t.m = m; t.P = P; t.general = !fast; t.K = K; t.nbl = nbl; t.nbr = nbr; t.wb = wb;
-    t.fac.assign(4 * m * 5, 0.0);
-    t.spk.assign(4 * m * 4, 0.0);
-    t.lhs.assign(4 * m * 5, 0.0);
+    t.m = m; t.P = P; t.xr = xr; t.general = !fast; t.K = K; t.nbl = nbl; t.nbr = nbr; t.wb = wb;
+    t.fac.assign(4 * mt * 5, 0.0);
+    t.spk.assign(4 * mt * 4, 0.0);
+    t.lhs.assign(4 * mt * 5, 0.0);
     t.G.assign((size_t)P * 2 * K * 2 * K * P, 0.0);
     t.sp.assign(2 * NBMAX * WBMAX, 0.0);
 
@@ -700,7 +708,8 @@ int build_spike(const SystemDesc &s, SpikeTables &t, bool force_general)
     for (int ty = 0; ty < 4; ty++) {
         const int sg = rep[ty];
         if (sg < 0) continue;
-        const int r0 = sg * m;
+        const int r0 = sg * m;  // segments start every m rows
+        const int m = rows_of(ty);
         std::vector<double> l1(m, 0), l2(m, 0), u0(m, 0), u1(m, 0), u2(m, 0);
         auto Aloc = [&](int j, int o) -> double {  // band entry of row j (local), offset o, inside the block
             int jj = j + o;
@@ -718,9 +727,9 @@ int build_spike(const SystemDesc &s, SpikeTables &t, bool force_general)
             l1[j] = L1; l2[j] = L2; u0[j] = U0; u1[j] = U1; u2[j] = U2;
         }
         for (int j = 0; j < m; j++) {
-            double *F = &t.fac[(ty * m + j) * 5];
+            double *F = &t.fac[(ty * mt + j) * 5];
             F[0] = l1[j]; F[1] = l2[j]; F[2] = 1.0 / u0[j]; F[3] = u1[j] / u0[j]; F[4] = u2[j] / u0[j];
-            double *Lb = &t.lhs[(ty * m + j) * 5];
+            double *Lb = &t.lhs[(ty * mt + j) * 5];
             for (int o = 0; o < 5; o++) Lb[o] = s.lhs[r0 + j][o];
             // entries leaving the domain of a non-periodic system are zero by construction
         }
@@ -750,8 +759,8 @@ int build_spike(const SystemDesc &s, SpikeTables &t, bool force_general)
             solve_local(bv);
             solve_local(cv);
             for (int j = 0; j < m; j++) {
-                t.spk[(ty * m + j) * 4 + r] = bv[j];
-                t.spk[(ty * m + j) * 4 + 2 + r] = cv[j];
+                t.spk[(ty * mt + j) * 4 + r] = bv[j];
+                t.spk[(ty * mt + j) * 4 + 2 + r] = cv[j];
             }
         }
     }
@@ -776,11 +785,11 @@ int build_spike(const SystemDesc &s, SpikeTables &t, bool force_general)
         const int ty = seg_type(sg);
         const int pv = prev_of(sg), nx = next_of(sg);
         for (int q = 0; q < 2 * K; q++) {
-            const int j = q < K ? q : m - 2 * K + q;  // local row of this unknown
+            const int j = q < K ? q : rows_of(ty) - 2 * K + q;  // local row of this unknown
             const int row = sg * 2 * K + q;
             for (int r = 0; r < K; r++) {
-                if (pv >= 0) R[row * NR + pv * 2 * K + K + r] += t.spk[(ty * m + j) * 4 + r];
-                if (nx >= 0) R[row * NR + nx * 2 * K + r] += t.spk[(ty * m + j) * 4 + 2 + r];
+                if (pv >= 0) R[row * NR + pv * 2 * K + K + r] += t.spk[(ty * mt + j) * 4 + r];
+                if (nx >= 0) R[row * NR + nx * 2 * K + r] += t.spk[(ty * mt + j) * 4 + 2 + r];
             }
         }
     }

@@ -77,13 +77,14 @@ void dense(const SystemDesc &s, double *A, double *B);
 // 2 last (non-periodic right end), 3 single segment with both ends.
 
 struct SpikeTables {
-    int m = 0, P = 0;        // segment length and count (P*m == n_sys)
+    int m = 0, P = 0;        // segment length and count (P*m + xr == n_sys)
+    int xr = 0;              // extra rows of the last segment (0, or 1 for 32 P + 1 non-periodic rows)
     bool general = false;    // single-pencil kernel (any n): P == 1, m == n_sys
     int K = 1, nbl = 0, nbr = 0, wb = 0;
-    // per segment type t, row j (0..m-1):
-    std::vector<double> fac;   // [4][m][5]: l1, l2, inv, u1*inv, u2*inv
-    std::vector<double> spk;   // [4][m][4]: V0, V1, W0, W1
-    std::vector<double> lhs;   // [4][m][5]: band of A (offsets -2..2)
+    // per segment type t, row j (0..m+xr-1; types 0 and 1 use the first m):
+    std::vector<double> fac;   // [4][m+xr][5]: l1, l2, inv, u1*inv, u2*inv
+    std::vector<double> spk;   // [4][m+xr][4]: V0, V1, W0, W1
+    std::vector<double> lhs;   // [4][m+xr][5]: band of A (offsets -2..2)
     std::vector<double> G;     // [P][2K][2K*P]: rows of R^-1 giving yL[0..K), yR[0..K)
     std::vector<double> sp;    // [2][NBMAX][WBMAX]: closure rows (left: cols 0..WBMAX-1, right: cols n_in-WBMAX..n_in-1)
     int D32 = 0, D64 = 0;      // periodic: segment offsets +-D whose coupling exceeds 1e-9 / 1e-18 of the largest
@@ -96,8 +97,8 @@ struct SpikeTables {
 
 int build_spike(const SystemDesc &s, SpikeTables &t, bool force_general = false);
 
-// Choose (m, P) for a system of n rows; returns false when only the general
-// kernel applies.
-bool choose_partition(int n_sys, int K, int nb, int &m, int &P);
+// Choose (m, P, xr) for a system of n rows; returns false when only the
+// general kernel applies.
+bool choose_partition(int n_sys, int K, int nb, bool periodic, int &m, int &P, int &xr);
 
 }  // namespace pdo

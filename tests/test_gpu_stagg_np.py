@@ -35,8 +35,12 @@ def _run(o, meth, op, n, flags, n1, n2, is_complex, dtype, prec, cuda, seed):
     return rel_l2(g, ref)
 
 
+# n = 64 and 256: the edge systems (n + 1 rows) run on the register path with
+# one extra row in the last segment; n = 63: edge systems of 64 rows on the
+# register path, cell systems on the one-lane-per-pencil kernel; n = 40: both
+# on the latter
 @pytest.mark.parametrize("combo", list(PADE6))
-@pytest.mark.parametrize("n", [64, 63, 40])
+@pytest.mark.parametrize("n", [256, 64, 63, 40])
 def test_stagg_np_fp64(cuda, combo, n):
     import torch
     te, be, ts, bs = PADE6[combo]
@@ -59,6 +63,9 @@ def test_stagg_np_fp32(cuda, combo, prec):
     flags = te * O.TOP_EVEN + be * O.BOT_EVEN + ts * O.TOP_SIDED + bs * O.BOT_SIDED
     o = P.cd06stagg()
     assert o.init(n, DZ, bool(te), bool(be), bool(ts), bool(bs)) == 0
+    # every operator of a 128-cell grid is on the register path (4 segments)
+    for kind in range(10, 18):
+        assert o.partition(kind) == (32, 4)
     for i, (meth, op) in enumerate(METHODS):
         err = _run(o, meth, op, n, flags, 70, 3, False, torch.float32, prec, cuda, i)
         print(combo, prec, meth, err)

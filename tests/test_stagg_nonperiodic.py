@@ -142,9 +142,13 @@ def test_library_system_matches_oracle(combo, n):
         assert np.abs(M - Mo).max() <= 1e-12 * max(1, np.abs(Mo).max()), (combo, kind)
 
 
-@pytest.mark.parametrize("combo", ["SS", "OE", "ES"])
-def test_library_partitioned_tables(combo):
-    n, dz = 64, 0.11
+# n = 64 and 256 cells: the edge systems (ddz_C2E, ddz_E2E, d2dz2_E2E,
+# InterpZ_C2E) have n + 1 rows, which the register path holds as n/32
+# segments with one row more in the last
+@pytest.mark.parametrize("n", [64, 256])
+@pytest.mark.parametrize("combo", ["SS", "OE", "ES", "EE", "OO"])
+def test_library_partitioned_tables(combo, n):
+    dz = 0.11
     te, be, ts, bs = PADE6[combo]
     o = _np_obj(n, dz, te, be, ts, bs)
     for op, kind in KIND.items():
@@ -153,6 +157,8 @@ def test_library_partitioned_tables(combo):
         ref = O.stagg_np(op, f.reshape(1, 1, -1), dz, _flags(te, be, ts, bs), n).ravel()
         for general in (False, True):
             t = S.tables(o, kind, 0, 0, general)
+            if not general:
+                assert (t["m"], t["P"], t["xr"]) == (32, n // 32, A.shape[0] - n), (combo, kind)
             d = B @ f
             for r, v in S.closure_rows(t, f, B.shape[1]).items():
                 assert abs(v - d[r]) <= 1e-10 * (1 + abs(d[r]))
