@@ -155,6 +155,27 @@ int pdo_cd06stagg_ddz_E2E(pdo_op h, int prec, const void *fE, void *dfE, int n1,
 
 int pdo_op_destroy(pdo_op h);
 
+/* Allocation-free apply.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same kernels and results
+ * as the entry points above:
+ *   kind   0 dd (first derivative), 1 d2d, 2 filter, 10..17 the staggered
+ *          operators (numbering of pdo_op_dense); dir 1 x, 2 y, 3 z with the
+ *          layouts of dd1 / dd2 / dd3 (staggered operators: dir 3, and na
+ *          counts real words, 2 n1 for complex data);
+ *   bc1, bcn as in the cd10 / cf90 calls (ignored by cd06 and cd06stagg).
+ * It calls no hipMalloc or hipFree.  The operator variant (handle, kind, bc1,
+ * bcn) must have been built by one earlier call of the ordinary entry point,
+ * otherwise PDO_ERR_ARG.  Sizes outside the register path (n not a multiple
+ * of 32 rows, more than 16 segments, non-periodic x pencils) use a work area
+ * of n_sys * na * nb FP64 words held in a fixed device array of
+ * PDO_NOALLOC_SCRATCH_BYTES (one per device, kept until the process ends);
+ * larger inputs return PDO_ERR_ARG.  Every call on one device shares that
+ * array: calls that overlap in time on one device (concurrent streams) race. */
+#define PDO_NOALLOC_SCRATCH_BYTES (32u << 20)
+int pdo_op_apply_noalloc(pdo_op h, int kind, int dir, int prec, const void *f, void *out, int na, int nb, int bc1,
+                         int bcn, void *stream);
+
 /* Host-side description of one operator variant (no device work): the dense
  * system A y = B f it solves, A n x n and B n x n_in, row-major FP64.  kind:
  * 0 first derivative, 1 second derivative, 2 filter, 10.. staggered

@@ -130,6 +130,20 @@ class _Op:
         _check(lib().pdo_op_dense(self._handle(), kind, bc1, bcn, A.ctypes.data_as(dp), B.ctypes.data_as(dp)))
         return A, B
 
+    def apply_noalloc(self, kind, dir, f, out, na, nb, bc1=0, bcn=0, prec="f32_refine", stream=None):
+        """pdo_op_apply_noalloc: the operator `kind` (numbering of dense())
+        along `dir` with no device allocation; the variant must have been
+        built by one ordinary call first.  na counts real words (2 n1 for
+        complex staggered data)."""
+        ns, ni = ctypes.c_int(), ctypes.c_int()
+        _check(lib().pdo_op_dims(self._handle(), kind, ctypes.byref(ns), ctypes.byref(ni)))
+        # edge outputs of the staggered operators hold n + 1 points
+        n_out = self.n + 1 if kind in (11, 13, 15, 17) else ns.value
+        if _reals(f) < ni.value * na * nb or _reals(out) < n_out * na * nb:
+            raise ValueError("buffer smaller than the declared shape")
+        return _check(lib().pdo_op_apply_noalloc(self._handle(), kind, dir, _prec(f, prec), _ptr(f), _ptr(out), na,
+                                                 nb, bc1, bcn, _stream(stream)))
+
     def partition(self, kind, bc1=0, bcn=0):
         m, P = ctypes.c_int(), ctypes.c_int()
         _check(lib().pdo_op_partition(self._handle(), kind, bc1, bcn, ctypes.byref(m), ctypes.byref(P)))

@@ -86,6 +86,7 @@ SIGNATURES = {
     "pdo_cd06stagg_ddz_C2C": (_STAGG, _c_int),
     "pdo_cd06stagg_ddz_E2E": (_STAGG, _c_int),
     "pdo_op_destroy": ([_vp], _c_int),
+    "pdo_op_apply_noalloc": ([_vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp], _c_int),
     "pdo_op_dense": ([_vp, _c_int, _c_int, _c_int, _pdbl, _pdbl], _c_int),
     "pdo_op_dims": ([_vp, _c_int, _pint, _pint], _c_int),
     "pdo_op_partition": ([_vp, _c_int, _c_int, _c_int, _pint, _pint], _c_int),
