@@ -135,6 +135,14 @@ def main():
     if roofline is not None:
         roofline["traffic"] = committed_traffic(roofline["kernel"])
 
+    # the mixed path's error of each filter direction (standalone cf90 passes)
+    errs.update(filter_errors(P, n, u64[0], u32[0], stream))
+    del g64, o64, g32, o32, rhs64, rhs32
+    torch.cuda.empty_cache()
+
+    # the wall-bounded 512 x 512 x 256 box: per-operator times and errors
+    wall = None if args.no_breakdown else wall_box(P, dev, stream, args.steps)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline()
@@ -168,11 +176,90 @@ def main():
         "fp32_store_rel_l2_vs_fp64": {k: float("%.3e" % v) for k, v in errs_s.items()},
         "roofline": roofline,
         "pass_ms": breakdown,
+        "wall_box_512x512x256": wall,
         "hbm_probe": probe,
         "cpu_baseline": cpu,
         "wall_s_timed_fp32r": round(wall32, 4),
     }
     print(json.dumps(line))
+
+
+def filter_errors(P, n, f64, f32, stream):
+    """Relative L2 error of the FP32+refine cf90 filter along x, y and z
+    against the FP64 one, on the first velocity component."""
+    import torch
+    c = P.cf90()
+    c.init(n, True)
+    out = {}
+    for d, name in ((1, "cf90_x"), (2, "cf90_y"), (3, "cf90_z")):
+        a = torch.empty_like(f64)
+        b = torch.empty_like(f32)
+        getattr(c, "filter%d" % d)(f64, a, n, n, prec="f64", stream=stream)
+        getattr(c, "filter%d" % d)(f32, b, n, n, prec="f32_refine", stream=stream)
+        torch.cuda.synchronize()
+        out[name] = rel_l2(b, a)
+        del a, b
+    return out
+
+
+def wall_box(P, dev, stream, steps):
+    """The wall-bounded box of 512 x 512 x 256 cells (periodic x, y; walls in
+    z; log-law mean profile plus the seeded turbulence): time per launch of
+    each operator in FP32+refine and FP64 (HIP events on the launch stream),
+    the achieved algorithmic GB/s (read the input field, write the output
+    field), and the FP32+refine error against FP64."""
+    import math
+
+    import torch
+    from padeops_amd.synthetic import wall_bounded
+    nx, nz = 512, 256
+    dz = 2 * math.pi / nx
+    F64 = dict(zip(("uC", "uE", "wE"), wall_bounded(nx, nx, nz, 8.0, 1234, torch.float64, dev)))
+    F32 = {k: v.float() for k, v in F64.items()}
+    c10z = P.cd10()
+    c10z.init(nz, dz, False, 0, 0)
+    c10x = P.cd10()
+    c10x.init(nx, dz, True, 0, 0)
+    sES = P.cd06stagg()
+    sES.init(nz, dz, True, True, False, True)
+    sOO = P.cd06stagg()
+    sOO.init(nz, dz, False, False, False, False)
+    ops = {
+        "cd10_dd3_walls": ("uC", nz, lambda f, g, p: c10z.dd3(f, g, nx, nx, 0, 0, prec=p, stream=stream)),
+        "cd10_d2d3_walls": ("uC", nz, lambda f, g, p: c10z.d2d3(f, g, nx, nx, 0, 0, prec=p, stream=stream)),
+        "stagg_ddz_C2E": ("uC", nz + 1, lambda f, g, p: sES.ddz_C2E(f, g, nx, nx, prec=p, stream=stream)),
+        "stagg_InterpZ_C2E": ("uC", nz + 1, lambda f, g, p: sES.InterpZ_C2E(f, g, nx, nx, prec=p, stream=stream)),
+        "stagg_ddz_E2C": ("uE", nz, lambda f, g, p: sES.ddz_E2C(f, g, nx, nx, prec=p, stream=stream)),
+        "stagg_d2dz2_E2E": ("uE", nz + 1, lambda f, g, p: sES.d2dz2_E2E(f, g, nx, nx, prec=p, stream=stream)),
+        "stagg_InterpZ_E2C_w": ("wE", nz, lambda f, g, p: sOO.InterpZ_E2C(f, g, nx, nx, prec=p, stream=stream)),
+        "cd10_dd1_periodic": ("uC", nz, lambda f, g, p: c10x.dd1(f, g, nx, nz, 0, 0, prec=p, stream=stream)),
+        "cd10_dd2_periodic": ("uC", nz, lambda f, g, p: c10x.dd2(f, g, nx, nz, 0, 0, prec=p, stream=stream)),
+    }
+    res = {}
+    for name, (key, nout, fn) in ops.items():
+        row = {}
+        outs = {}
+        for prec, F, dt in (("f32_refine", F32, torch.float32), ("f64", F64, torch.float64)):
+            f = F[key]
+            g = torch.empty(nx * nx * nout, dtype=dt, device=dev)
+            for _ in range(2):
+                fn(f, g, prec)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(steps):
+                fn(f, g, prec)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / steps
+            nbytes = (f.numel() + g.numel()) * f.element_size()
+            row["ms_" + prec] = round(ms, 5)
+            row["GBps_" + prec] = round(nbytes / (ms * 1e-3) / 1e9, 1)
+            outs[prec] = g
+        row["fp32r_rel_l2_vs_fp64"] = float("%.3e" % rel_l2(outs["f32_refine"], outs["f64"]))
+        res[name] = row
+        del outs
+    return res
 
 
 def pass_calls(P, n, f32, stream):
@@ -222,13 +309,21 @@ def pass_breakdown(P, n, f32, stream, steps):
         e1.record()
         torch.cuda.synchronize()
         res[name] = e0.elapsed_time(e1) / steps
-    share = {k: res[k] * uses[k] for k in res if k != "poisson_5pass"}
+    # time per evaluation of each pass type, the Poisson solve included; the
+    # roofline entry describes the largest share
+    share = {k: res[k] * uses[k] for k in res}
     dom = max(share, key=share.get)
-    algo = 2 * N * 4  # read N words + write N words (FP32)
+    # algorithmic words per point of one launch: a compact pass reads and
+    # writes one field; the Poisson solve's five passes move 10 (r2c x, y
+    # forward, fused z, y inverse, c2r x: 2 each, the half spectrum counted
+    # as one word per point)
+    words = 10 if dom == "poisson_5pass" else 2
+    algo = words * N * 4
     ach = algo / (res[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": PEAK_HBM, "unit": "GB/s",
             "frac": round(ach / PEAK_HBM, 4), "traffic": None,
-            "bytes_per_launch": algo, "avg_launch_ms": round(res[dom], 5)}
+            "bytes_per_launch": algo, "avg_launch_ms": round(res[dom], 5),
+            "ms_per_eval_by_pass": {k: round(v, 4) for k, v in share.items()}}
     return {k: round(v, 5) for k, v in res.items()}, roof
 
 

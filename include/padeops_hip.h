@@ -301,6 +301,25 @@ int pdo_duidxj_init(pdo_duidxj *h, pdo_spectral spC, pdo_spectral spE, pdo_pade6
 int pdo_duidxj_compute(pdo_duidxj h, int prec, const pdo_duidxj_fields *fields, void *stream);
 int pdo_duidxj_destroy(pdo_duidxj h);
 
+/* ---- igrid%interp_PrimitiveVars ------------------------------------------
+ * replaces igrid.F90:1423-1463 on the pdo_duidxj object (its spectral
+ * objects, Pade6stagg and boundary kinds bc[0..5] = uBC_bottom, uBC_top,
+ * vBC_bottom, vBC_top, wBC_bottom, wBC_top):
+ *   whatC = interpz_E2C(what), wC = ifft(whatC)    (wBC)
+ *   uEhat = interpz_C2E(uhat), uE = ifft(uEhat)    (uBC), v likewise
+ *   with That != NULL: TEhat = interpz_C2E(That) (TBC_bottom, TBC_top); when
+ *   botBC_Temp == 0 its bottom plane is zeroed and its mean mode set to
+ *   Tsurf nx ny (:1449-1454); TE = ifft(TEhat).
+ * Cell spectra (nx/2+1, ny, nz), edge spectra (nx/2+1, ny, nz+1); real
+ * fields (nx, ny, nz) / (nx, ny, nz+1). */
+typedef struct {
+    const void *uhat, *vhat, *what, *That;
+    void *whatC, *wC, *uEhat, *uE, *vEhat, *vE, *TEhat, *TE;
+    int TBC_bottom, TBC_top, botBC_Temp;
+    double Tsurf;
+} pdo_interp_fields;
+int pdo_igrid_interp_primitive(pdo_duidxj h, int prec, const pdo_interp_fields *fields, void *stream);
+
 /* ---- SGS eddy-viscosity models (src/incompressible/sgs_models) -------------
  * pdo_sgs_kernel: get_Sij_from_duidxj (eddyViscosity.F90:1-21) fused with the
  * model kernel, model 0 Smagorinsky |S| = sqrt(2 S_ij S_ij)
@@ -323,6 +342,92 @@ int pdo_dynproc_compute(pdo_dynproc h, int prec, const void *u, const void *v, c
                         const void *vhat, const void *what, const void *Sij, const void *Dsgs, double *lambda_1d,
                         void *stream);
 int pdo_dynproc_destroy(pdo_dynproc h);
+
+/* ---- sgs_igrid%getRHS_SGS (sgsmod_igrid.F90:204-266) ------------------------
+ * with getTauSGS (:156-201) for the eddy-viscosity models:
+ *   S_ij and the kernel on cells (duidxjC) and on edges (duidxjE, kernel only
+ *   with explicit_edge = explicitCalcEdgeEddyViscosity);
+ *   multiply_by_model_constant (eddyViscosity.F90:74-103): lambda != NULL
+ *   (the dynamic procedure's planar coefficient, device, nz values, e.g. from
+ *   pdo_dynproc_compute) scales nuC and interpolates it to the edges;
+ *   otherwise nuC (and with explicit_edge nuE) take cmodelC / cmodelE per
+ *   plane (host arrays of nz / nz+1, copied at init) or cglobal when NULL,
+ *   and without explicit_edge nuE = interpz_C2E(nuC) clipped at zero;
+ *   tau_ij = -2 nu S_ij (11, 12, 22, 33 on cells; 13, 23 on edges);
+ *   urhs -= d/dx tau11 + d/dy tau12 + ddz_E2C(tau13),
+ *   vrhs -= d/dx tau12 + d/dy tau22 + ddz_E2C(tau23),
+ *   wrhs -= ddz_C2E(tau33) + d/dx tau13 + d/dy tau23
+ *   (x, y spectral; z Pade6stagg with (0, 0) ends).
+ * urhs, vrhs: (nx/2+1, ny, nz) complex; wrhs: (nx/2+1, ny, nz+1).  prec at
+ * init sizes the work buffers; compute must use the same precision.
+ * pdo_sgsrhs_fields: device pointers of the last tau (cells: 4 fields of
+ * nx ny nz, edges: 2 of nx ny (nz+1)), nu_sgs_C and nu_sgs_E. */
+typedef struct pdo_sgsrhs_s *pdo_sgsrhs;
+typedef struct {
+    int model;        /* 0 Smagorinsky, 2 AMD */
+    int explicit_edge;
+    double camd_x, camd_y, camd_z;
+    double cglobal;
+    const double *cmodelC, *cmodelE;
+} pdo_sgs_params;
+int pdo_sgsrhs_init(pdo_sgsrhs *h, pdo_spectral spC, pdo_spectral spE, pdo_pade6stagg dz, const pdo_sgs_params *p,
+                    int prec);
+int pdo_sgsrhs_compute(pdo_sgsrhs h, int prec, const void *duidxjC, const void *duidxjE, const double *lambda,
+                       void *urhs, void *vrhs, void *wrhs, void *stream);
+int pdo_sgsrhs_fields(pdo_sgsrhs h, const void **tauC, const void **tauE, const void **nuC, const void **nuE);
+int pdo_sgsrhs_destroy(pdo_sgsrhs h);
+
+/* ---- igrid time advancement, periodic in z ----------------------------------
+ * replaces the time loop of type(igrid) (src/incompressible/igrid.F90) for
+ * domains periodic in x, y and z (the forced isotropic turbulence setup):
+ *   timeAdvance :1056-1076 with scheme 1 TVD_RK3 :1105-1170 or 2 SSP_RK45
+ *     :1172-1299 (timeSteppingScheme); the stage-1 projection is taken
+ *     explicitly, as with fastCalcPressure = .false.
+ *   populate_rhs :1793-1912: AddNonLinearTerm_Rot :1497-1566,
+ *     addViscousTerm :1914-1959 (Re > 0), getRHS_SGS (use_sgs; the
+ *     pdo_sgsrhs model constants, no dynamic procedure, which the reference
+ *     does not support with periodic z either), HIT_shell_forcing
+ *     forcingIsotropic.F90:123-314 (use_forcing)
+ *   project_and_prep :1961-1992: dealiasFields (periodic-z masks), the
+ *     periodic PadePoisson projection, inverse transforms,
+ *     interp_PrimitiveVars, compute_duidxj
+ *   compute_deltaT :1374-1400 (CFL > 0; otherwise the fixed dt)
+ * The forcing draws its random wave vectors with this library's generator
+ * (splitmix64 of seed, draw counter, wave and variable), not the Fortran
+ * runtime's random_number.  set_velocity: physical u, v (nx, ny, nz) and w
+ * (nx, ny, nz+1, edges), then project_and_prep; time and step restart at 0.
+ * pdo_igrid_field: 0 u, 1 v, 2 wC, 3 w, 4 uhat, 5 vhat, 6 what, 7 duidxjC,
+ * 8 duidxjE, 9 urhs, 10 vrhs, 11 wrhs, 12 uE, 13 vE (device pointers of the
+ * object's storage, current after each step). */
+typedef struct pdo_igrid_s *pdo_igrid;
+typedef struct {
+    int nx, ny, nz;
+    double Lx, Ly, Lz;
+    double Re;       /* <= 0: inviscid */
+    int scheme;      /* 1 TVD-RK3, 2 SSP-RK45 */
+    double CFL;      /* > 0: dt from the CFL number each step */
+    double dt;       /* fixed time step when CFL <= 0 */
+    int use_sgs;
+    pdo_sgs_params sgs;
+    int use_forcing;
+    double kmin, kmax, eps_amplitude;
+    int nwaves;
+    unsigned long long seed;
+} pdo_igrid_params;
+int pdo_igrid_init(pdo_igrid *h, const pdo_igrid_params *p, int prec);
+int pdo_igrid_set_velocity(pdo_igrid h, const void *u, const void *v, const void *w, void *stream);
+int pdo_igrid_step(pdo_igrid h, int nsteps, void *stream);
+int pdo_igrid_field(pdo_igrid h, int which, const void **ptr);
+int pdo_igrid_time(pdo_igrid h, double *tsim, long *step, double *dt);
+/* Restart files (igrid.F90:2719-2800): dump writes dir/RESTART_RunRR_u.TTTTTT,
+ * _v, _w (the physical fields of the current step as raw FP64 in Fortran
+ * order, global arrays as decomp_2d_write_one leaves them; w on the nz+1
+ * edges) and _info.TTTTTT (tsim as one "(100g15.5)" record), TTTTTT the step
+ * number, RR the run id.  read loads such a set (FP64 data converted to the
+ * object's precision), prepares the stage and sets tsim and the step count. */
+int pdo_igrid_dump_restart(pdo_igrid h, const char *dir, int rid, void *stream);
+int pdo_igrid_read_restart(pdo_igrid h, const char *dir, int rid, long tid, void *stream);
+int pdo_igrid_destroy(pdo_igrid h);
 
 /* ---- learned SGS closure -----------------------------------------------------
  * No counterpart in the reference: the pointwise MLP closure of this build

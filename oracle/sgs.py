@@ -58,3 +58,45 @@ def dynamic_procedure(sp, dx, dy, deltaRat, u, v, w, uhat, vhat, what, S, Dsgs):
     mf = 1.0 / (nx * ny)
     a = np.maximum(num.sum(axis=(0, 1)) * mf, 0.0)
     return a / (den.sum(axis=(0, 1)) * mf + 1e-18)
+
+
+def rhs_sgs(spC, spE, dz, periodic, duidxjC, duidxjE, urhs, vrhs, wrhs, model=0, explicit_edge=False,
+            cglobal=1.0, cmodelC=None, cmodelE=None, camd=(1.0, 1.0, 1.0), lam=None):
+    """sgs_igrid%getRHS_SGS (sgsmod_igrid.F90:204-266) with getTauSGS
+    (:156-201), multiply_by_model_constant and interpolate_eddy_viscosity
+    (eddyViscosity.F90:74-122) for the eddy-viscosity models (0 Smagorinsky,
+    2 AMD).  Returns (urhs, vrhs, wrhs, tau dict, nuC, nuE)."""
+    from .spectral import pade6_apply
+    from . import STG_D1_C2E, STG_D1_E2C, STG_IN_C2E
+    nx, ny, nz = spC.shape
+    SC, SE = sij(duidxjC), sij(duidxjE)
+
+    def kernel(g, S):
+        return smagorinsky(S) if model == 0 else amd(g, S, *camd)
+    nuC = kernel(duidxjC, SC)
+    interp = lambda f: pade6_apply(STG_IN_C2E, f, dz, nz, 0, 0, periodic).real  # noqa: E731
+    if lam is not None:
+        nuC = nuC * np.asarray(lam)[None, None, :]
+        nuE = interp(nuC)
+    else:
+        nuC = nuC * (np.asarray(cmodelC)[None, None, :] if cmodelC is not None else cglobal)
+        if explicit_edge:
+            nuE = kernel(duidxjE, SE) * (np.asarray(cmodelE)[None, None, :] if cmodelE is not None else cglobal)
+        else:
+            nuE = np.maximum(interp(nuC), 0.0)
+    tau = {"11": -2 * nuC * SC[..., 0], "12": -2 * nuC * SC[..., 1], "13": -2 * nuE * SE[..., 2],
+           "22": -2 * nuC * SC[..., 3], "23": -2 * nuE * SE[..., 4], "33": -2 * nuC * SC[..., 5]}
+    ddz = lambda op, f: pade6_apply(op, f, dz, nz, 0, 0, periodic)  # noqa: E731
+    urhs = urhs - spC.mtimes_ik(spC.fft(tau["11"]), 1)
+    vrhs = vrhs - spC.mtimes_ik(spC.fft(tau["22"]), 2)
+    wrhs = wrhs - ddz(STG_D1_C2E, spC.fft(tau["33"]))
+    f12 = spC.fft(tau["12"])
+    vrhs = vrhs - spC.mtimes_ik(f12, 1)
+    urhs = urhs - spC.mtimes_ik(f12, 2)
+    f13 = spE.fft(tau["13"])
+    urhs = urhs - ddz(STG_D1_E2C, f13)
+    wrhs = wrhs - spE.mtimes_ik(f13, 1)
+    f23 = spE.fft(tau["23"])
+    vrhs = vrhs - ddz(STG_D1_E2C, f23)
+    wrhs = wrhs - spE.mtimes_ik(f23, 2)
+    return urhs, vrhs, wrhs, tau, nuC, nuE

@@ -200,3 +200,28 @@ def duidxj(spC, spE, dz, periodic, bc, inviscid, uhat, vhat, whatC, what, uEhat,
         RC[..., k - 1] = spC.ifft(HC[..., k - 1])
     out.update(duidxjC=RC, duidxjE=RE, duidxjChat=HC, duidxjEhat=HE)
     return out
+
+
+def interp_primitive(spC, spE, dz, periodic, bc, uhat, vhat, what, That=None, Tbc=(1, 1), botBC_Temp=1,
+                     Tsurf=0.0):
+    """igrid%interp_PrimitiveVars (igrid.F90:1423-1463): w to cells, u, v (and
+    T) to edges with Pade6stagg, then the 2D inverse transforms.  bc = (uBC_bottom,
+    uBC_top, vBC_bottom, vBC_top, wBC_bottom, wBC_top, ...)."""
+    nx, ny, nz = spC.shape
+    P6 = lambda op, f, b, t: pade6_apply(op, f, dz, nz, b, t, periodic)  # noqa: E731
+    out = {}
+    out["whatC"] = P6(STG_IN_E2C, what, bc[4], bc[5])
+    out["wC"] = spC.ifft(out["whatC"])
+    out["uEhat"] = P6(STG_IN_C2E, uhat, bc[0], bc[1])
+    out["uE"] = spE.ifft(out["uEhat"])
+    out["vEhat"] = P6(STG_IN_C2E, vhat, bc[2], bc[3])
+    out["vE"] = spE.ifft(out["vEhat"])
+    if That is not None:
+        TE = P6(STG_IN_C2E, That, Tbc[0], Tbc[1])
+        if botBC_Temp == 0:
+            # :1449-1454: the bottom plane holds only the surface mean
+            TE[:, :, 0] = 0.0
+            TE[0, 0, 0] = Tsurf * nx * ny
+        out["TEhat"] = TE
+        out["TE"] = spE.ifft(TE)
+    return out

@@ -786,6 +786,168 @@ class duidxj:
             pass
 
 
+def interp_PrimitiveVars(op, uhat, vhat, what, whatC, wC, uEhat, uE, vEhat, vE, That=None, TEhat=None, TE=None,
+                         TBC_bottom=1, TBC_top=1, botBC_Temp=1, Tsurf=0.0, prec="f32_refine", stream=None):
+    """igrid%interp_PrimitiveVars (igrid.F90:1423-1463) on a duidxj object
+    (its spectral objects, Pade6stagg and boundary kinds)."""
+    F = _lib.InterpFields()
+    for name, t in (("uhat", uhat), ("vhat", vhat), ("what", what), ("That", That), ("whatC", whatC), ("wC", wC),
+                    ("uEhat", uEhat), ("uE", uE), ("vEhat", vEhat), ("vE", vE), ("TEhat", TEhat), ("TE", TE)):
+        if t is not None:
+            setattr(F, name, _ptr(t).value)
+    F.TBC_bottom, F.TBC_top, F.botBC_Temp, F.Tsurf = int(TBC_bottom), int(TBC_top), int(botBC_Temp), float(Tsurf)
+    return _check(lib().pdo_igrid_interp_primitive(op._h, _prec(uhat, prec),
+                                                    ctypes.cast(ctypes.pointer(F), ctypes.c_void_p),
+                                                    _stream(stream)))
+
+
+class sgsRHS:
+    """sgs_igrid%getRHS_SGS (sgsmod_igrid.F90:204-266) for the eddy-viscosity
+    models: model "smagorinsky" or "amd"; cmodelC / cmodelE: per-plane model
+    constants (sequences of nz / nz+1) or None for cglobal."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def init(self, spC, spE, dzop, model="smagorinsky", explicit_edge=False, camd=(1.0, 1.0, 1.0), cglobal=1.0,
+             cmodelC=None, cmodelE=None, prec="f32_refine"):
+        self._keep = (spC, spE, dzop)
+        p = _lib.SgsParams()
+        p.model = {"smagorinsky": 0, "amd": 2}[model]
+        p.explicit_edge = int(explicit_edge)
+        p.camd_x, p.camd_y, p.camd_z = camd
+        p.cglobal = cglobal
+        arrs = []
+        for name, c in (("cmodelC", cmodelC), ("cmodelE", cmodelE)):
+            if c is not None:
+                a = (ctypes.c_double * len(c))(*[float(x) for x in c])
+                arrs.append(a)
+                setattr(p, name, ctypes.cast(a, ctypes.POINTER(ctypes.c_double)))
+        return _check(lib().pdo_sgsrhs_init(ctypes.byref(self._h), spC._h, spE._h, dzop._h,
+                                            ctypes.cast(ctypes.pointer(p), ctypes.c_void_p), PREC[prec]))
+
+    def compute(self, duidxjC, duidxjE, urhs, vrhs, wrhs, lambda_1d=None, prec="f32_refine", stream=None):
+        """urhs, vrhs, wrhs -= the divergence of the SGS stress (in place)."""
+        return _check(lib().pdo_sgsrhs_compute(self._h, _prec(duidxjC, prec), _ptr(duidxjC), _ptr(duidxjE),
+                                               _ptr(lambda_1d), _ptr(urhs), _ptr(vrhs), _ptr(wrhs),
+                                               _stream(stream)))
+
+    def fields(self):
+        """device pointers (tauC, tauE, nuC, nuE) of the last compute"""
+        out = [ctypes.c_void_p() for _ in range(4)]
+        _check(lib().pdo_sgsrhs_fields(self._h, *[ctypes.byref(o) for o in out]))
+        return [o.value for o in out]
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_sgsrhs_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class igrid:
+    """Time advancement of type(igrid) (src/incompressible/igrid.F90) on one
+    GPU for domains periodic in x, y and z: TVD-RK3 (scheme 1) or SSP-RK45
+    (scheme 2) stages of populate_rhs (rotational nonlinear term, viscous
+    term, optional SGS stress and HIT forcing) and project_and_prep."""
+
+    FIELDS = {"u": 0, "v": 1, "wC": 2, "w": 3, "uhat": 4, "vhat": 5, "what": 6, "duidxjC": 7, "duidxjE": 8,
+              "urhs": 9, "vrhs": 10, "wrhs": 11, "uE": 12, "vE": 13}
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def init(self, nx, ny, nz, Lx, Ly, Lz, Re=0.0, scheme=2, CFL=0.0, dt=0.0, sgs=None, forcing=None,
+             prec="f64"):
+        """sgs: dict(model, explicit_edge, camd, cglobal) or None; forcing:
+        dict(kmin, kmax, nwaves, eps, seed) or None."""
+        p = _lib.IgridParams()
+        p.nx, p.ny, p.nz, p.Lx, p.Ly, p.Lz = nx, ny, nz, Lx, Ly, Lz
+        p.Re, p.scheme, p.CFL, p.dt = Re, scheme, CFL, dt
+        if sgs is not None:
+            p.use_sgs = 1
+            p.sgs.model = {"smagorinsky": 0, "amd": 2}[sgs.get("model", "smagorinsky")]
+            p.sgs.explicit_edge = int(sgs.get("explicit_edge", False))
+            p.sgs.camd_x, p.sgs.camd_y, p.sgs.camd_z = sgs.get("camd", (1.0, 1.0, 1.0))
+            p.sgs.cglobal = sgs.get("cglobal", 1.0)
+        if forcing is not None:
+            p.use_forcing = 1
+            p.kmin, p.kmax = forcing["kmin"], forcing["kmax"]
+            p.nwaves, p.eps_amplitude, p.seed = forcing["nwaves"], forcing["eps"], forcing["seed"]
+        self.shape = (nx, ny, nz)
+        self.prec = prec
+        return _check(lib().pdo_igrid_init(ctypes.byref(self._h), ctypes.cast(ctypes.pointer(p), ctypes.c_void_p),
+                                           PREC[prec]))
+
+    def set_velocity(self, u, v, w, stream=None):
+        """physical u, v on cells (nx, ny, nz) and w on edges (nx, ny, nz+1)"""
+        return _check(lib().pdo_igrid_set_velocity(self._h, _ptr(u), _ptr(v), _ptr(w), _stream(stream)))
+
+    def step(self, nsteps=1, stream=None):
+        return _check(lib().pdo_igrid_step(self._h, nsteps, _stream(stream)))
+
+    def field(self, name):
+        """a torch view of one of the object's fields (FIELDS)"""
+        import torch
+        ptr = ctypes.c_void_p()
+        _check(lib().pdo_igrid_field(self._h, self.FIELDS[name], ctypes.byref(ptr)))
+        nx, ny, nz = self.shape
+        nxh = nx // 2 + 1
+        sizes = {"u": nx * ny * nz, "v": nx * ny * nz, "wC": nx * ny * nz, "w": nx * ny * (nz + 1),
+                 "uE": nx * ny * (nz + 1), "vE": nx * ny * (nz + 1), "duidxjC": 9 * nx * ny * nz,
+                 "duidxjE": 9 * nx * ny * (nz + 1), "uhat": nxh * ny * nz, "vhat": nxh * ny * nz,
+                 "what": nxh * ny * (nz + 1), "urhs": nxh * ny * nz, "vrhs": nxh * ny * nz,
+                 "wrhs": nxh * ny * (nz + 1)}
+        spectral_ = name in ("uhat", "vhat", "what", "urhs", "vrhs", "wrhs")
+        real = torch.float64 if self.prec == "f64" else torch.float32
+        cplx = torch.complex128 if self.prec == "f64" else torch.complex64
+        dt = cplx if spectral_ else real
+        n = sizes[name]
+        W = torch.empty((), dtype=dt).element_size()
+        # wrap the device memory without copying (the object owns it)
+        holder = _DeviceView(ptr.value, n * W)
+        t = torch.as_tensor(holder, device="cuda").view(dt)
+        return t[:n]
+
+    def dumpRestartFile(self, outputdir, runID, stream=None):
+        """RESTART_RunRR_{u,v,w,info}.TTTTTT of igrid.F90:2771-2800"""
+        return _check(lib().pdo_igrid_dump_restart(self._h, str(outputdir).encode(), int(runID), _stream(stream)))
+
+    def readRestartFile(self, inputdir, tid, rid, stream=None):
+        """igrid.F90:2719-2769"""
+        return _check(lib().pdo_igrid_read_restart(self._h, str(inputdir).encode(), int(rid), int(tid),
+                                                   _stream(stream)))
+
+    def time(self):
+        t, s, d = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
+        _check(lib().pdo_igrid_time(self._h, ctypes.byref(t), ctypes.byref(s), ctypes.byref(d)))
+        return t.value, s.value, d.value
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_igrid_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class _DeviceView:
+    """__cuda_array_interface__ of a byte range of device memory owned by a
+    library object (torch.as_tensor wraps it without a copy)."""
+
+    def __init__(self, ptr, nbytes):
+        self.__cuda_array_interface__ = {"shape": (nbytes,), "typestr": "|u1", "data": (ptr, False), "version": 3}
+
+
 def sgs_kernel(duidxj, Sij, nu_sgs, npts, model="smagorinsky", camd=(0.0, 0.0, 0.0), prec="f32_refine",
                stream=None):
     """get_Sij_from_duidxj + get_smagorinsky_kernel / get_amd_kernel (sgs_models)."""

@@ -108,6 +108,19 @@ SIGNATURES = {
     "pdo_dynproc_init": ([_pvp, _vp, _c_double], _c_int),
     "pdo_dynproc_compute": ([_vp, _c_int] + [_vp] * 9 + [_vp], _c_int),
     "pdo_dynproc_destroy": ([_vp], _c_int),
+    "pdo_igrid_interp_primitive": ([_vp, _c_int, _vp, _vp], _c_int),
+    "pdo_sgsrhs_init": ([_pvp, _vp, _vp, _vp, _vp, _c_int], _c_int),
+    "pdo_sgsrhs_compute": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "pdo_sgsrhs_fields": ([_vp, _pvp, _pvp, _pvp, _pvp], _c_int),
+    "pdo_sgsrhs_destroy": ([_vp], _c_int),
+    "pdo_igrid_init": ([_pvp, _vp, _c_int], _c_int),
+    "pdo_igrid_set_velocity": ([_vp, _vp, _vp, _vp, _vp], _c_int),
+    "pdo_igrid_step": ([_vp, _c_int, _vp], _c_int),
+    "pdo_igrid_field": ([_vp, _c_int, _pvp], _c_int),
+    "pdo_igrid_time": ([_vp, _pdbl, ctypes.POINTER(ctypes.c_long), _pdbl], _c_int),
+    "pdo_igrid_destroy": ([_vp], _c_int),
+    "pdo_igrid_dump_restart": ([_vp, ctypes.c_char_p, _c_int, _vp], _c_int),
+    "pdo_igrid_read_restart": ([_vp, ctypes.c_char_p, _c_int, ctypes.c_long, _vp], _c_int),
     "pdo_mlp_nparam": ([], _c_int),
     "pdo_mlp_create": ([_pvp, ctypes.c_long], _c_int),
     "pdo_mlp_set_params": ([_vp, _vp], _c_int),
@@ -146,6 +159,30 @@ class DuidxjFields(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in (
         "uhat", "vhat", "whatC", "what", "uEhat", "vEhat", "duidxjC", "duidxjE", "duidxjChat", "duidxjEhat",
         "d2wdz2hatE", "d2udz2hatC", "d2vdz2hatC")]
+
+
+class InterpFields(ctypes.Structure):
+    """pdo_interp_fields of include/padeops_hip.h"""
+    _fields_ = [(name, ctypes.c_void_p) for name in (
+        "uhat", "vhat", "what", "That", "whatC", "wC", "uEhat", "uE", "vEhat", "vE", "TEhat", "TE")] + [
+        ("TBC_bottom", ctypes.c_int), ("TBC_top", ctypes.c_int), ("botBC_Temp", ctypes.c_int),
+        ("Tsurf", ctypes.c_double)]
+
+
+class SgsParams(ctypes.Structure):
+    """pdo_sgs_params of include/padeops_hip.h"""
+    _fields_ = [("model", ctypes.c_int), ("explicit_edge", ctypes.c_int), ("camd_x", ctypes.c_double),
+                ("camd_y", ctypes.c_double), ("camd_z", ctypes.c_double), ("cglobal", ctypes.c_double),
+                ("cmodelC", ctypes.POINTER(ctypes.c_double)), ("cmodelE", ctypes.POINTER(ctypes.c_double))]
+
+
+class IgridParams(ctypes.Structure):
+    """pdo_igrid_params of include/padeops_hip.h"""
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("Lx", ctypes.c_double),
+                ("Ly", ctypes.c_double), ("Lz", ctypes.c_double), ("Re", ctypes.c_double), ("scheme", ctypes.c_int),
+                ("CFL", ctypes.c_double), ("dt", ctypes.c_double), ("use_sgs", ctypes.c_int), ("sgs", SgsParams),
+                ("use_forcing", ctypes.c_int), ("kmin", ctypes.c_double), ("kmax", ctypes.c_double),
+                ("eps_amplitude", ctypes.c_double), ("nwaves", ctypes.c_int), ("seed", ctypes.c_ulonglong)]
 
 
 class LibraryMissing(ImportError):

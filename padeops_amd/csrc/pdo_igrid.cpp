@@ -5,9 +5,12 @@
 // GPU the reference's y <-> z transposes are identities, so the z operators
 // work on the spectral arrays in place of transposed copies.
 #include <cstring>
+#include <vector>
 
 #include "../../include/padeops_hip.h"
+#include "pdo_igrid.hpp"
 #include "pdo_ops.hpp"
+#include "pdo_poisson.hpp"
 #include "pdo_spectral.hpp"
 
 namespace pdo {
@@ -115,6 +118,222 @@ int pdo_duidxj_compute(pdo_duidxj h, int prec, const pdo_duidxj_fields *F, void 
 }
 
 int pdo_duidxj_destroy(pdo_duidxj h)
+{
+    delete h;
+    return 0;
+}
+
+// igrid%interp_PrimitiveVars (igrid.F90:1423-1463): w to cells, u and v (and
+// T) to edges through Pade6stagg, each followed by the 2D inverse transform
+int pdo_igrid_interp_primitive(pdo_duidxj h, int prec, const pdo_interp_fields *F, void *stream)
+{
+    if (!h || !F || !F->uhat || !F->vhat || !F->what || !F->whatC || !F->wC || !F->uEhat || !F->uE || !F->vEhat ||
+        !F->vE || (F->That && (!F->TEhat || !F->TE))) {
+        set_error("invalid argument");
+        return PDO_ERR_ARG;
+    }
+    const int mode = fft_mode(prec);
+    if (mode < 0) { set_error("unknown precision mode"); return PDO_ERR_ARG; }
+    SpectralObject &C = *h->spC;
+    const int nxh = C.nxh, ny = C.ny;
+    const int *bc = h->bc;
+    int st;
+    // Step 1: w -> wC (wBC_bottom, wBC_top)
+    if ((st = pdo_pade6stagg_apply(h->dz, 14, bc[4], bc[5], prec, F->what, F->whatC, nxh, ny, 1, stream))) return st;
+    if ((st = pdo_spectral_ifft(h->spC, prec, F->whatC, F->wC, 0, stream))) return st;
+    // Steps 2, 3: u -> uE, v -> vE
+    if ((st = pdo_pade6stagg_apply(h->dz, 15, bc[0], bc[1], prec, F->uhat, F->uEhat, nxh, ny, 1, stream))) return st;
+    if ((st = pdo_spectral_ifft(h->spE, prec, F->uEhat, F->uE, 0, stream))) return st;
+    if ((st = pdo_pade6stagg_apply(h->dz, 15, bc[2], bc[3], prec, F->vhat, F->vEhat, nxh, ny, 1, stream))) return st;
+    if ((st = pdo_spectral_ifft(h->spE, prec, F->vEhat, F->vE, 0, stream))) return st;
+    // Step 4: T -> TE; a prescribed surface temperature (botBC_Temp == 0)
+    // replaces the bottom edge plane by its mean Tsurf (:1449-1454)
+    if (F->That) {
+        if ((st = pdo_pade6stagg_apply(h->dz, 15, F->TBC_bottom, F->TBC_top, prec, F->That, F->TEhat, nxh, ny, 1,
+                                       stream)))
+            return st;
+        if (F->botBC_Temp == 0) {
+            hipError_t e = launch_surface_plane(mode, F->TEhat, (long)nxh * ny, F->Tsurf * (double)C.nx * C.ny,
+                                                (hipStream_t)stream);
+            if (e != hipSuccess) return hip_fail(e, "surface temperature");
+        }
+        if ((st = pdo_spectral_ifft(h->spE, prec, F->TEhat, F->TE, 0, stream))) return st;
+    }
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// sgs_igrid%getRHS_SGS (sgsmod_igrid.F90:204-266) with getTauSGS (:156-201)
+// for the eddy-viscosity models
+// ---------------------------------------------------------------------------
+namespace pdo {
+
+struct SgsRhsObject {
+    pdo_spectral spC = nullptr, spE = nullptr;
+    pdo_pade6stagg dz = nullptr;
+    pdo_sgs_params p{};
+    double *d_cC = nullptr, *d_cE = nullptr;  // planar model constants (cmodelC, cmodelE), device
+    size_t nC = 0, nE = 0, sC = 0, sE = 0;     // points (cells, edges), spectral values (cells, edges)
+    int wprec = -1;                            // precision the work buffers are sized for
+    std::vector<void *> bufs;
+    void *SC = nullptr, *SE = nullptr, *nuC = nullptr, *nuE = nullptr, *tauC = nullptr, *tauE = nullptr;
+    void *cb1 = nullptr, *cb2 = nullptr;
+    void free_bufs()
+    {
+        for (void *b : bufs) (void)hipFree(b);
+        bufs.clear();
+    }
+    ~SgsRhsObject()
+    {
+        free_bufs();
+        if (d_cC) (void)hipFree(d_cC);
+        if (d_cE) (void)hipFree(d_cE);
+    }
+    int alloc(int prec)
+    {
+        if (wprec == prec) return 0;
+        free_bufs();
+        const size_t W = prec == PDO_F64 ? 8 : 4;
+        const size_t sizes[8] = {6 * nC * W, 6 * nE * W, nC * W, nE * W, 4 * nC * W, 2 * nE * W, 2 * sC * W, 2 * sE * W};
+        void **dst[8] = {&SC, &SE, &nuC, &nuE, &tauC, &tauE, &cb1, &cb2};
+        for (int i = 0; i < 8; i++) {
+            hipError_t e = hipMalloc(dst[i], sizes[i]);
+            if (e != hipSuccess) return hip_fail(e, "hipMalloc(sgs work)");
+            bufs.push_back(*dst[i]);
+        }
+        wprec = prec;
+        return 0;
+    }
+};
+
+}  // namespace pdo
+
+struct pdo_sgsrhs_s : public pdo::SgsRhsObject {};
+
+extern "C" {
+
+int pdo_sgsrhs_init(pdo_sgsrhs *h, pdo_spectral spC, pdo_spectral spE, pdo_pade6stagg dzop, const pdo_sgs_params *p,
+                    int prec)
+{
+    if (!h || !spC || !spE || !dzop || !p) { set_error("invalid argument"); return PDO_ERR_ARG; }
+    *h = nullptr;
+    if (p->model != 0 && p->model != 2) { set_error("eddy-viscosity model: 0 Smagorinsky or 2 AMD"); return PDO_ERR_ARG; }
+    if (spE->nx != spC->nx || spE->ny != spC->ny || spE->nz != spC->nz + 1) {
+        set_error("sgs: the edge spectral object must have nz+1 planes of the cell object's grid");
+        return PDO_ERR_ARG;
+    }
+    if (prec_mode(prec) < 0) { set_error("unknown precision mode"); return PDO_ERR_ARG; }
+    auto *o = new pdo_sgsrhs_s();
+    o->spC = spC; o->spE = spE; o->dz = dzop; o->p = *p;
+    o->p.cmodelC = nullptr; o->p.cmodelE = nullptr;
+    const int nx = spC->nx, ny = spC->ny, nz = spC->nz;
+    o->nC = (size_t)nx * ny * nz; o->nE = (size_t)nx * ny * (nz + 1);
+    o->sC = (size_t)spC->nxh * ny * nz; o->sE = (size_t)spC->nxh * ny * (nz + 1);
+    hipError_t e = hipSuccess;
+    if (p->cmodelC) {
+        e = hipMalloc(&o->d_cC, nz * sizeof(double));
+        if (e == hipSuccess) e = hipMemcpy(o->d_cC, p->cmodelC, nz * sizeof(double), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && p->cmodelE) {
+        e = hipMalloc(&o->d_cE, (nz + 1) * sizeof(double));
+        if (e == hipSuccess) e = hipMemcpy(o->d_cE, p->cmodelE, (nz + 1) * sizeof(double), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) { delete o; return hip_fail(e, "sgs model constants"); }
+    if (int st = o->alloc(prec)) { delete o; return st; }
+    *h = o;
+    return 0;
+}
+
+int pdo_sgsrhs_compute(pdo_sgsrhs h, int prec, const void *duidxjC, const void *duidxjE, const double *lambda,
+                       void *urhs, void *vrhs, void *wrhs, void *stream)
+{
+    if (!h || !duidxjC || !duidxjE || !urhs || !vrhs || !wrhs) { set_error("invalid argument"); return PDO_ERR_ARG; }
+    const int mode = fft_mode(prec);
+    if (mode < 0) { set_error("unknown precision mode"); return PDO_ERR_ARG; }
+    if (h->wprec != prec) {
+        set_error("sgs: the object's work buffers were sized for another precision (pass it to init)");
+        return PDO_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    SpectralObject &C = *h->spC, &E = *h->spE;
+    const int nx = C.nx, ny = C.ny, nz = C.nz, nxh = C.nxh;
+    const long nxy = (long)nx * ny;
+    const pdo_sgs_params &p = h->p;
+    int r;
+#define PDO_TRY(x)                \
+    do {                          \
+        if ((r = (x))) return r;  \
+    } while (0)
+#define PDO_HIP(x, what)                                        \
+    do {                                                        \
+        hipError_t ee = (x);                                    \
+        if (ee != hipSuccess) return hip_fail(ee, what);        \
+    } while (0)
+    // Step 0/1: S_ij and the model kernel on cells; on edges S_ij and, with
+    // explicitCalcEdgeEddyViscosity, the kernel too
+    const bool edge_kernel = p.explicit_edge && !lambda;
+    PDO_TRY(pdo_sgs_kernel(prec, p.model, duidxjC, h->SC, h->nuC, (long)h->nC, p.camd_x, p.camd_y, p.camd_z, stream));
+    PDO_TRY(pdo_sgs_kernel(prec, p.model, duidxjE, h->SE, edge_kernel ? h->nuE : nullptr, (long)h->nE, p.camd_x,
+                           p.camd_y, p.camd_z, stream));
+    // Step 3: multiply_by_model_constant (eddyViscosity.F90:74-103)
+    if (lambda) {
+        // the dynamic procedure's planar coefficient, then C2E interpolation
+        PDO_TRY(pdo_sgs_scale(prec, h->nuC, nxy, nz, lambda, 0.0, stream));
+        PDO_TRY(pdo_pade6stagg_apply(h->dz, 15, 0, 0, prec, h->nuC, h->nuE, nx, ny, 0, stream));
+    } else {
+        PDO_TRY(pdo_sgs_scale(prec, h->nuC, nxy, nz, h->d_cC, p.cglobal, stream));
+        if (p.explicit_edge) {
+            PDO_TRY(pdo_sgs_scale(prec, h->nuE, nxy, nz + 1, h->d_cE, p.cglobal, stream));
+        } else {
+            // interpolate_eddy_viscosity(checknegative = .true.)
+            PDO_TRY(pdo_pade6stagg_apply(h->dz, 15, 0, 0, prec, h->nuC, h->nuE, nx, ny, 0, stream));
+            PDO_HIP(launch_clip_negative(mode, h->nuE, (long)h->nE, st), "clip");
+        }
+    }
+    // tau_ij = -2 nu S_ij
+    PDO_HIP(launch_tau(mode, h->nuC, h->SC, h->nuE, h->SE, h->tauC, h->tauE, (long)h->nC, (long)h->nE, st), "tau");
+    const size_t W = prec == PDO_F64 ? 8 : 4;
+    auto tc = [&](int q) { return (const void *)((const char *)h->tauC + (size_t)q * h->nC * W); };
+    auto te = [&](int q) { return (const void *)((const char *)h->tauE + (size_t)q * h->nE * W); };
+    const long nsC = (long)h->sC, nsE = (long)h->sE;
+    // ddx(tau11) -> urhs; ddy(tau22) -> vrhs
+    PDO_TRY(pdo_spectral_fft(h->spC, prec, tc(0), h->cb1, stream));
+    PDO_HIP(launch_sub(mode, urhs, h->cb1, C.d_k1, nxh, ny, nsC, 1, st), "rhs");
+    PDO_TRY(pdo_spectral_fft(h->spC, prec, tc(2), h->cb1, stream));
+    PDO_HIP(launch_sub(mode, vrhs, h->cb1, C.d_k2, nxh, ny, nsC, 2, st), "rhs");
+    // ddz(tau33) -> wrhs (ddz_C2E with (0, 0) ends)
+    PDO_TRY(pdo_spectral_fft(h->spC, prec, tc(3), h->cb1, stream));
+    PDO_TRY(pdo_pade6stagg_apply(h->dz, 11, 0, 0, prec, h->cb1, h->cb2, nxh, ny, 1, stream));
+    PDO_HIP(launch_sub(mode, wrhs, h->cb2, nullptr, nxh, ny, nsE, 0, st), "rhs");
+    // ddx(tau12) -> vrhs, ddy(tau12) -> urhs
+    PDO_TRY(pdo_spectral_fft(h->spC, prec, tc(1), h->cb1, stream));
+    PDO_HIP(launch_sub(mode, vrhs, h->cb1, C.d_k1, nxh, ny, nsC, 1, st), "rhs");
+    PDO_HIP(launch_sub(mode, urhs, h->cb1, C.d_k2, nxh, ny, nsC, 2, st), "rhs");
+    // ddz(tau13) -> urhs, ddx(tau13) -> wrhs; ddz(tau23) -> vrhs, ddy(tau23) -> wrhs
+    for (int q = 0; q < 2; q++) {
+        PDO_TRY(pdo_spectral_fft(h->spE, prec, te(q), h->cb2, stream));
+        PDO_TRY(pdo_pade6stagg_apply(h->dz, 10, 0, 0, prec, h->cb2, h->cb1, nxh, ny, 1, stream));
+        PDO_HIP(launch_sub(mode, q == 0 ? urhs : vrhs, h->cb1, nullptr, nxh, ny, nsC, 0, st), "rhs");
+        PDO_HIP(launch_sub(mode, wrhs, h->cb2, q == 0 ? E.d_k1 : E.d_k2, nxh, ny, nsE, q + 1, st), "rhs");
+    }
+#undef PDO_TRY
+#undef PDO_HIP
+    return 0;
+}
+
+int pdo_sgsrhs_fields(pdo_sgsrhs h, const void **tauC, const void **tauE, const void **nuC, const void **nuE)
+{
+    if (!h) return PDO_ERR_ARG;
+    if (tauC) *tauC = h->tauC;
+    if (tauE) *tauE = h->tauE;
+    if (nuC) *nuC = h->nuC;
+    if (nuE) *nuE = h->nuE;
+    return 0;
+}
+
+int pdo_sgsrhs_destroy(pdo_sgsrhs h)
 {
     delete h;
     return 0;

@@ -34,3 +34,35 @@ def turbulence(n, k0, seed, dtype, device):
     del uh, kd
     rms = torch.sqrt((u[0] ** 2 + u[1] ** 2 + u[2] ** 2).mean() / 3.0)
     return [(c / rms).contiguous().view(-1).to(dtype) for c in (u[2], u[1], u[0])]
+
+
+def wall_bounded(nx, ny, nz, k0, seed, dtype, device, z0=1e-4, kappa=0.4):
+    """Fields of the wall-bounded box (nx, ny, nz) cells, periodic in x and y
+    with side 2 pi, non-periodic in z over [0, Lz] with dz = 2 pi / nx (so
+    Lz = nz dz): the log-law mean profile U(z) = ln(z / z0) / kappa plus the
+    isotropic turbulence of turbulence(nx, k0, seed) restricted to its first
+    planes.  Returns (uC, uE, wE): u at the nz cell centres z = (k + 1/2) dz,
+    u at the nz + 1 edges z = k dz (U taken as ln(max(z, z0) / z0) / kappa,
+    zero at the wall), and the w component at the edges; flat tensors in the
+    Fortran layout (x fastest).  nx == ny and nz < nx are required."""
+    import math
+
+    import torch
+    assert nx == ny and nz < nx
+    n = nx
+    u, _, w = turbulence(n, k0, seed, torch.float64, device)
+    u = u.view(n, n, n)  # torch view (z, y, x) of the Fortran (x, y, z) layout
+    w = w.view(n, n, n)
+    dz = 2 * math.pi / n
+    zc = (torch.arange(nz, device=device, dtype=torch.float64) + 0.5) * dz
+    ze = torch.arange(nz + 1, device=device, dtype=torch.float64) * dz
+    Uc = torch.log(zc / z0) / kappa
+    Ue = torch.log(torch.clamp(ze, min=z0) / z0) / kappa
+    # turbulence on cells: the average of the two neighbouring planes
+    # (cell k lies between planes k and k + 1 of the periodic field)
+    tc = 0.5 * (u[:nz] + u[1:nz + 1])
+    uC = (tc + Uc.view(nz, 1, 1)).contiguous().view(-1).to(dtype)
+    uE = (u[:nz + 1] + Ue.view(nz + 1, 1, 1)).contiguous().view(-1).to(dtype)
+    wE = w[:nz + 1].contiguous().view(-1).to(dtype)
+    del u, w
+    return uC, uE, wE

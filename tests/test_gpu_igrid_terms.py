@@ -1,0 +1,150 @@
+"""GPU parity of igrid%interp_PrimitiveVars (igrid.F90:1423-1463) and of
+sgs_igrid%getRHS_SGS (sgsmod_igrid.F90:204-266, with getTauSGS :156-201)
+against the oracle compositions (oracle/spectral.py interp_primitive,
+oracle/sgs.py rhs_sgs), wall-bounded and periodic in z.
+
+nz = 32 cells: the 33-row edge systems run on the register path with an
+extra row.  FP64 relative L2 <= 1e-12 per output; the FP32+refine path
+against the oracle on the same FP32-rounded inputs <= 5e-6 (FP32 transforms)."""
+import numpy as np
+import pytest
+
+import padeops_amd as P
+from fields import rel_l2
+from oracle.sgs import rhs_sgs
+from oracle.spectral import Spectral, interp_primitive
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NZ = 32, 16, 32
+DX, DY, DZ = 2 * np.pi / NX, 2 * np.pi / NY, 1.0 / NZ
+NXH = NX // 2 + 1
+
+
+def _dev(a, cuda, single):
+    import torch
+    flat = np.ascontiguousarray(np.asarray(a).ravel(order="F"))
+    if np.iscomplexobj(flat):
+        return torch.from_numpy(flat.astype(np.complex64 if single else np.complex128)).to(cuda)
+    return torch.from_numpy(flat.astype(np.float32 if single else np.float64)).to(cuda)
+
+
+def _host(t, shape):
+    import torch
+    torch.cuda.synchronize()
+    x = t.detach().cpu()
+    x = x.to(torch.complex128) if x.is_complex() else x.double()
+    return x.numpy().reshape(shape, order="F")
+
+
+def _r32(a):
+    a = np.asarray(a)
+    if np.iscomplexobj(a):
+        return a.real.astype(np.float32).astype(np.float64) + 1j * a.imag.astype(np.float32).astype(np.float64)
+    return a.astype(np.float32).astype(np.float64)
+
+
+def _objects(periodic, bc, inviscid=True):
+    sc, se = P.spectral(), P.spectral()
+    assert sc.init(NX, NY, NZ, DX, DY, DZ) == 0 and se.init(NX, NY, NZ + 1, DX, DY, DZ) == 0
+    d6 = P.Pade6stagg()
+    assert d6.init(NZ, DZ, periodic) == 0
+    op = P.duidxj()
+    assert op.init(sc, se, d6, bc, inviscid) == 0
+    return sc, se, d6, op
+
+
+BCS = [(1, 1, 1, 1, -1, -1, -1, -1, -1, -1, 1, 1), (0, 1, 0, 1, -1, -1, 0, -1, 0, -1, 1, 1)]
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("prec", ["f64", "f32_refine"])
+@pytest.mark.parametrize("bc", BCS)
+@pytest.mark.parametrize("temp", [None, (0, 1, 0, 1.7), (1, 1, 1, 0.0)])
+def test_interp_primitive(cuda, periodic, prec, bc, temp):
+    import torch
+    single = prec != "f64"
+    spC, spE = Spectral(NX, NY, NZ, DX, DY, DZ), Spectral(NX, NY, NZ + 1, DX, DY, DZ)
+    rng = np.random.default_rng(7 + int(periodic))
+    cf = lambda planes: rng.standard_normal((NXH, NY, planes)) + 1j * rng.standard_normal((NXH, NY, planes))  # noqa
+    ins = dict(uhat=cf(NZ), vhat=cf(NZ), what=cf(NZ + 1))
+    if temp is not None:
+        ins["That"] = cf(NZ)
+    if single:
+        ins = {k: _r32(v) for k, v in ins.items()}
+    kw = {}
+    if temp is not None:
+        kw = dict(Tbc=(temp[0], temp[1]), botBC_Temp=temp[2], Tsurf=temp[3])
+    ref = interp_primitive(spC, spE, DZ, periodic, bc, ins["uhat"], ins["vhat"], ins["what"], ins.get("That"), **kw)
+    sc, se, d6, op = _objects(periodic, bc)
+    ct, rt = (torch.complex64, torch.float32) if single else (torch.complex128, torch.float64)
+    shapes = dict(whatC=((NXH, NY, NZ), ct), wC=((NX, NY, NZ), rt), uEhat=((NXH, NY, NZ + 1), ct),
+                  uE=((NX, NY, NZ + 1), rt), vEhat=((NXH, NY, NZ + 1), ct), vE=((NX, NY, NZ + 1), rt))
+    if temp is not None:
+        shapes.update(TEhat=((NXH, NY, NZ + 1), ct), TE=((NX, NY, NZ + 1), rt))
+    outs = {k: torch.full((int(np.prod(s)),), float("nan"), dtype=t, device=cuda) for k, (s, t) in shapes.items()}
+    args = {k: _dev(v, cuda, single) for k, v in ins.items()}
+    targs = {}
+    if temp is not None:
+        targs = dict(TBC_bottom=temp[0], TBC_top=temp[1], botBC_Temp=temp[2], Tsurf=temp[3])
+    assert P.interp_PrimitiveVars(op, prec=prec, **args, **outs, **targs) == 0
+    bar = 1e-12 if not single else 5e-6
+    for k, (s, _) in shapes.items():
+        err = rel_l2(_host(outs[k], s), ref[k])
+        print(k, prec, periodic, err)
+        assert err <= bar, k
+
+
+# (model, explicit_edge, model constants: "global" / "planar" / "dynamic")
+SGS_CASES = [("smagorinsky", False, "global"), ("smagorinsky", True, "planar"), ("amd", False, "planar"),
+             ("amd", True, "global"), ("smagorinsky", False, "dynamic")]
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("prec", ["f64", "f32_refine"])
+@pytest.mark.parametrize("case", SGS_CASES, ids=["-".join(map(str, c)) for c in SGS_CASES])
+def test_sgs_rhs(cuda, periodic, prec, case):
+    import torch
+    model, explicit_edge, consts = case
+    single = prec != "f64"
+    spC, spE = Spectral(NX, NY, NZ, DX, DY, DZ), Spectral(NX, NY, NZ + 1, DX, DY, DZ)
+    rng = np.random.default_rng(11 + int(periodic) + 3 * SGS_CASES.index(case))
+    cf = lambda planes: rng.standard_normal((NXH, NY, planes)) + 1j * rng.standard_normal((NXH, NY, planes))  # noqa
+    gC = rng.standard_normal((NX, NY, NZ, 9))
+    gE = rng.standard_normal((NX, NY, NZ + 1, 9))
+    ins = dict(gC=gC, gE=gE, urhs=cf(NZ), vrhs=cf(NZ), wrhs=cf(NZ + 1))
+    if single:
+        ins = {k: _r32(v) for k, v in ins.items()}
+    cmC = 0.02 + 0.01 * rng.random(NZ)
+    cmE = 0.02 + 0.01 * rng.random(NZ + 1)
+    lam = 0.05 + 0.1 * rng.random(NZ)
+    camd = (0.3, 0.2, 0.25)
+    kw = dict(model=0 if model == "smagorinsky" else 2, explicit_edge=explicit_edge, camd=camd)
+    if consts == "global":
+        kw["cglobal"] = 0.017
+    elif consts == "planar":
+        kw.update(cmodelC=cmC, cmodelE=cmE)
+    else:
+        kw["lam"] = lam
+    ur, vr, wr, tau, nuC, nuE = rhs_sgs(spC, spE, DZ, periodic, ins["gC"], ins["gE"], ins["urhs"], ins["vrhs"],
+                                        ins["wrhs"], **kw)
+    sc, se, d6, _ = _objects(periodic, BCS[0])
+    obj = P.sgsRHS()
+    ikw = dict(model=model, explicit_edge=explicit_edge, camd=camd, prec=prec)
+    if consts == "global":
+        ikw["cglobal"] = 0.017
+    elif consts == "planar":
+        ikw.update(cmodelC=cmC, cmodelE=cmE)
+    assert obj.init(sc, se, d6, **ikw) == 0
+    d = {k: _dev(v, cuda, single) for k, v in ins.items()}
+    lam_d = torch.from_numpy(lam).to(cuda) if consts == "dynamic" else None
+    assert obj.compute(d["gC"], d["gE"], d["urhs"], d["vrhs"], d["wrhs"], lambda_1d=lam_d, prec=prec) == 0
+    torch.cuda.synchronize()
+    bar = 1e-12 if not single else 5e-6
+    for name, t, r, s in (("urhs", d["urhs"], ur, (NXH, NY, NZ)), ("vrhs", d["vrhs"], vr, (NXH, NY, NZ)),
+                          ("wrhs", d["wrhs"], wr, (NXH, NY, NZ + 1))):
+        err = rel_l2(_host(t, s), r)
+        print(name, case, prec, periodic, err)
+        assert err <= bar, name
+    # the stored stress and eddy viscosities are exposed for inspection
+    assert all(p for p in obj.fields())
