@@ -80,6 +80,11 @@ int pdo_malloc(void **ptr, size_t bytes);
 int pdo_free(void *ptr);
 int pdo_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int pdo_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* device-to-device copy, asynchronous on stream (nothing when dst == src):
+ * the one-GPU transpose_x_to_y / y_to_z / z_to_y / y_to_x of 2DECOMP&FFT
+ * (decomp_2d.f90 transpose_*), whose pencils on one process hold the whole
+ * box in the same (i, j, k) order */
+int pdo_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 
 /* ---- cd06: 6th-order collocated compact derivative ----------------------
  * replaces type(cd06) of src/derivatives/cd06.F90:
@@ -208,6 +213,12 @@ int pdo_op_tables(pdo_op h, int kind, int bc1, int bcn, int general, int *meta, 
  * ny and nz powers of two in [8, 1024].  rhs and f are real (nx, ny, nz); rhs == f (in place) allowed. */
 int pdo_poisson_periodic_init(pdo_poisson *h, int nx, int ny, int nz, double dx, double dy, double dz, int modk_x,
                               int modk_y, int modk_z, int zero_policy);
+/* PoissonPeriodic%init with the Get_ModKx/y/z callbacks already applied
+ * (PoissonPeriodic.F90:188-204): kx[nx/2+1], ky[ny], kz[nz] are the
+ * wavenumbers of GetWaveNums (:226-242) after the callbacks, in 1/length
+ * (host arrays, copied).  The solve divides by kx^2 + ky^2 + kz^2. */
+int pdo_poisson_periodic_init_k(pdo_poisson *h, int nx, int ny, int nz, const double *kx, const double *ky,
+                                const double *kz, int zero_policy);
 int pdo_poisson_periodic_solve(pdo_poisson h, int prec, const void *rhs, void *f, void *stream);
 int pdo_poisson_destroy(pdo_poisson h);
 

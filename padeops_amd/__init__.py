@@ -330,6 +330,14 @@ class cd06stagg(_Op):
         return self._ap("pdo_cd06stagg_ddz_E2E", fE, dfE, n1, n2, self.n + 1, self.n + 1, is_complex, prec, stream)
 
 
+def _wavenums(n, d):
+    """GetWaveNums (PoissonPeriodic.F90:226-242): (-pi + i 2 pi / (n - n mod 2)) / d,
+    i = 0 .. n-1, then ifftshift."""
+    import numpy as np
+    t = (-np.pi + np.arange(n) * 2 * np.pi / (n - n % 2)) / d
+    return np.fft.ifftshift(t)
+
+
 class PoissonPeriodic:
     """type(PoissonPeriodic), src/utilities/PoissonPeriodic.F90 (dir_id = 1)."""
 
@@ -337,7 +345,25 @@ class PoissonPeriodic:
         self._h = ctypes.c_void_p()
 
     def init(self, dx, dy, dz, nx, ny, nz, modk=("fourier", "fourier", "fourier"), zero_policy=0):
+        """modk[d] names a built-in modified wavenumber ("fourier", "cd06",
+        "cd06stagg", "cd10") or is a Get_ModK callback: a callable taking the
+        wavenumbers times the spacing (numpy FP64) and returning their modified
+        values, applied as PoissonPeriodic.F90:188-204 does."""
+        import numpy as np
         self.shape = (nx, ny, nz)
+        self.destroy()
+        if any(callable(m) for m in modk):
+            ks = []
+            for n, d, m, cnt in ((nx, dx, modk[0], nx // 2 + 1), (ny, dy, modk[1], ny), (nz, dz, modk[2], nz)):
+                k = _wavenums(n, d)[:cnt]
+                if callable(m):
+                    k = np.ascontiguousarray(np.asarray(m(k * d), dtype=np.float64).reshape(cnt)) / d
+                elif (MODK[m] if isinstance(m, str) else int(m)) != 0:
+                    raise ValueError("mix callbacks only with Fourier directions")
+                ks.append(np.ascontiguousarray(k, dtype=np.float64))
+            dp = ctypes.POINTER(ctypes.c_double)
+            return _check(lib().pdo_poisson_periodic_init_k(ctypes.byref(self._h), nx, ny, nz,
+                                                            *[k.ctypes.data_as(dp) for k in ks], zero_policy))
         mk = [MODK[m] if isinstance(m, str) else int(m) for m in modk]
         return _check(lib().pdo_poisson_periodic_init(ctypes.byref(self._h), nx, ny, nz, dx, dy, dz, mk[0], mk[1],
                                                       mk[2], zero_policy))

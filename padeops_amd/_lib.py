@@ -29,6 +29,7 @@ SIGNATURES = {
     "pdo_free": ([_vp], _c_int),
     "pdo_memcpy_h2d": ([_vp, _vp, _sz], _c_int),
     "pdo_memcpy_d2h": ([_vp, _vp, _sz], _c_int),
+    "pdo_memcpy_d2d": ([_vp, _vp, _sz, _vp], _c_int),
     "pdo_cd06_init": ([_pvp, _c_int, _c_double, _c_int, _c_int, _c_int], _c_int),
     "pdo_cd06_dd1": (_APPLY, _c_int),
     "pdo_cd06_dd2": (_APPLY, _c_int),
@@ -93,6 +94,7 @@ SIGNATURES = {
     "pdo_op_tables": ([_vp, _c_int, _c_int, _c_int, _c_int, _pint, _pdbl, _pdbl, _pdbl, _pdbl], _c_int),
     "pdo_poisson_periodic_init": ([_pvp, _c_int, _c_int, _c_int, _c_double, _c_double, _c_double,
                                    _c_int, _c_int, _c_int, _c_int], _c_int),
+    "pdo_poisson_periodic_init_k": ([_pvp, _c_int, _c_int, _c_int, _pdbl, _pdbl, _pdbl, _c_int], _c_int),
     "pdo_poisson_periodic_solve": ([_vp, _c_int, _vp, _vp, _vp], _c_int),
     "pdo_poisson_destroy": ([_vp], _c_int),
     "pdo_pade6stagg_init": ([_pvp, _c_int, _c_double, _c_int], _c_int),

@@ -62,13 +62,16 @@ def build_fortran(verbose=False):
         return None
     work = os.path.join(OBJ, "fortran")
     os.makedirs(work, exist_ok=True)
+    dec = os.path.join(FORT, "padeops_hip_decomp.F90")
     mod = os.path.join(FORT, "padeops_hip_mod.F90")
     drv = os.path.join(FORT, "test_shim.F90")
     exe = os.path.join(FORT, "test_shim")
     steps = [
+        [fc, "-O2", "-module-dir", work, "-c", dec, "-o", os.path.join(work, "dec.o")],
         [fc, "-O2", "-module-dir", work, "-c", mod, "-o", os.path.join(work, "mod.o")],
         [fc, "-O2", "-module-dir", work, "-c", drv, "-o", os.path.join(work, "drv.o")],
-        [fc, os.path.join(work, "drv.o"), os.path.join(work, "mod.o"), "-L", HERE, "-lpadeops_hip",
+        [fc, os.path.join(work, "drv.o"), os.path.join(work, "mod.o"), os.path.join(work, "dec.o"), "-L", HERE,
+         "-lpadeops_hip",
          "-Wl,-rpath,$ORIGIN/..", "-o", exe],
     ]
     for cmd in steps:

@@ -334,6 +334,13 @@ int pdo_memcpy_d2h(void *dst, const void *src, size_t bytes)
     return e == hipSuccess ? 0 : hip_fail(e, "hipMemcpy");
 }
 
+int pdo_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (dst == src || bytes == 0) return 0;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+    return e == hipSuccess ? 0 : hip_fail(e, "hipMemcpyAsync");
+}
+
 static int make_op(pdo_op *h, int scheme, int n, double dx, int periodic, int bc1, int bcn)
 {
     if (!h) { set_error("null handle"); return PDO_ERR_ARG; }

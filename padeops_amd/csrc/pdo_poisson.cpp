@@ -130,34 +130,44 @@ PoissonObject::~PoissonObject()
 
 int PoissonObject::init(int nx_, int ny_, int nz_, double dx, double dy, double dz, int mx, int my, int mz, int zp)
 {
+    auto kp = [](int n, double d, int kind, int count) {
+        std::vector<double> k = wavenums(n, d), out(count);
+        for (int i = 0; i < count; i++) out[i] = modified_k(kind, k[i], d);
+        return out;
+    };
+    const std::vector<double> kx = kp(nx_, dx, mx, nx_ / 2 + 1), ky = kp(ny_, dy, my, ny_), kz = kp(nz_, dz, mz, nz_);
+    return init_k(nx_, ny_, nz_, kx.data(), ky.data(), kz.data(), zp);
+}
+
+int PoissonObject::init_k(int nx_, int ny_, int nz_, const double *kx, const double *ky, const double *kz, int zp)
+{
     if (!fft_pow2(nx_, 16, 1024) || !fft_pow2(ny_, 8, 1024) || !fft_pow2(nz_, 8, 1024)) {
         set_error("Poisson solver sizes must be powers of two in [16,1024] (x) / [8,1024] (y, z)");
         return PDO_ERR_ARG;
     }
+    if (!kx || !ky || !kz) { set_error("null wavenumber table"); return PDO_ERR_ARG; }
+    if (zp != 0 && zp != 1) { set_error("zero_policy must be 0 or 1"); return PDO_ERR_ARG; }
     nx = nx_; ny = ny_; nz = nz_;
     std::memset(&dev, 0, sizeof(dev));
     dev.nx = nx; dev.ny = ny; dev.nz = nz; dev.zero_policy = zp;
     const int nxh = nx / 2 + 1;
     dev.nxp = (nxh + 15) / 16 * 16;
-    auto k2 = [&](int n, double d, int kind, int count) {
-        std::vector<double> k = wavenums(n, d), out(count);
-        for (int i = 0; i < count; i++) {
-            const double kp = modified_k(kind, k[i], d);
-            out[i] = kp * kp;
-        }
+    auto sq = [](const double *k, int count) {
+        std::vector<double> out(count);
+        for (int i = 0; i < count; i++) out[i] = k[i] * k[i];
         return out;
     };
     double *p;
     int st;
-    if ((st = upload_vec(k2(nx, dx, mx, nxh), &p))) return st;
+    if ((st = upload_vec(sq(kx, nxh), &p))) return st;
     allocs.push_back(p); dev.kx2 = p;
-    if ((st = upload_vec(k2(ny, dy, my, ny), &p))) return st;
+    if ((st = upload_vec(sq(ky, ny), &p))) return st;
     allocs.push_back(p); dev.ky2 = p;
-    if ((st = upload_vec(k2(nz, dz, mz, nz), &p))) return st;
+    if ((st = upload_vec(sq(kz, nz), &p))) return st;
     allocs.push_back(p); dev.kz2 = p;
     {
-        std::vector<double> k = wavenums(nx, dx), kp(nxh);
-        for (int i = 0; i < nxh; i++) kp[i] = (i == 0 || i == nx / 2) ? 0.0 : modified_k(mx, k[i], dx);
+        std::vector<double> kp(nxh);
+        for (int i = 0; i < nxh; i++) kp[i] = (i == 0 || i == nx / 2) ? 0.0 : kx[i];
         if ((st = upload_vec(kp, &p))) return st;
         allocs.push_back(p); kpx = p;
     }
@@ -211,6 +221,18 @@ int pdo_poisson_periodic_init(pdo_poisson *h, int nx, int ny, int nz, double dx,
     *h = nullptr;
     auto *o = new pdo_poisson_s();
     int st = o->init(nx, ny, nz, dx, dy, dz, modk_x, modk_y, modk_z, zero_policy);
+    if (st) { delete o; return st; }
+    *h = o;
+    return 0;
+}
+
+int pdo_poisson_periodic_init_k(pdo_poisson *h, int nx, int ny, int nz, const double *kx, const double *ky,
+                                const double *kz, int zero_policy)
+{
+    if (!h) return PDO_ERR_ARG;
+    *h = nullptr;
+    auto *o = new pdo_poisson_s();
+    int st = o->init_k(nx, ny, nz, kx, ky, kz, zero_policy);
     if (st) { delete o; return st; }
     *h = o;
     return 0;

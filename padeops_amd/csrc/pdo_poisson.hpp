@@ -27,6 +27,8 @@ struct PoissonObject {
     size_t spec_bytes = 0;
     double *kpx = nullptr;  // x modified wavenumber k'(kx), zero at kx = 0 and nx/2 (device, nx/2+1)
     int init(int nx, int ny, int nz, double dx, double dy, double dz, int mx, int my, int mz, int zero_policy);
+    // kx (nx/2+1), ky (ny), kz (nz): the wavenumbers the solver divides by
+    int init_k(int nx, int ny, int nz, const double *kx, const double *ky, const double *kz, int zero_policy);
     int solve(int prec, const void *rhs, void *f, hipStream_t st);
     // p = Poisson(div + D_x ut) and ux = ut - D_x p with D_x the x derivative
     // whose modified wavenumber the solver uses (p may alias div)

@@ -1,7 +1,11 @@
 """The Fortran ISO_C_BINDING shim on the GPU: padeops_amd/fortran/test_shim
-(built by padeops_amd.build when amdflang is installed) calls cd06%dd1 and
-cf90%filter1 through the shim on device arrays and checks them against the
-scheme's closed forms and the reference's init codes."""
+(built by padeops_amd.build when amdflang is installed) drives every type of
+the shim on device arrays (cd06, cf90, cd06stagg periodic and non-periodic
+with its real and complex generics, PoissonPeriodic%init with a Get_ModKz
+callback, derivatives, filters, spectral, Pade6stagg, padepoisson and the
+decomp_info / transpose_* module) and checks each against a closed form
+(modified wavenumbers, single Fourier modes, constants) and the reference's
+init codes."""
 import os
 import subprocess
 
