@@ -68,6 +68,7 @@ typedef struct pdo_pade6stagg_s *pdo_pade6stagg;
 typedef struct pdo_duidxj_s *pdo_duidxj;
 typedef struct pdo_dynproc_s *pdo_dynproc;
 typedef struct pdo_mlp_s *pdo_mlp;
+typedef struct pdo_closure_s *pdo_closure;
 typedef struct pdo_xfilter_s *pdo_xfilter;
 typedef struct pdo_derivatives_s *pdo_derivatives;
 typedef struct pdo_filters_s *pdo_filters;
@@ -386,6 +387,12 @@ int pdo_sgsrhs_init(pdo_sgsrhs *h, pdo_spectral spC, pdo_spectral spE, pdo_pade6
 int pdo_sgsrhs_compute(pdo_sgsrhs h, int prec, const void *duidxjC, const void *duidxjE, const double *lambda,
                        void *urhs, void *vrhs, void *wrhs, void *stream);
 int pdo_sgsrhs_fields(pdo_sgsrhs h, const void **tauC, const void **tauE, const void **nuC, const void **nuE);
+/* replace the eddy-viscosity stress by a learned closure's (pdo_closure_stress
+ * on the cell and edge gradients, filter width delta, feature dratio); with
+ * fallback the eddy-viscosity stress stays where the closure's dissipation is
+ * negative.  c = NULL restores the eddy-viscosity model.  No reference
+ * counterpart. */
+int pdo_sgsrhs_set_closure(pdo_sgsrhs h, pdo_closure c, double delta, double dratio, int fallback);
 int pdo_sgsrhs_destroy(pdo_sgsrhs h);
 
 /* ---- igrid time advancement, periodic in z ----------------------------------
@@ -408,8 +415,8 @@ int pdo_sgsrhs_destroy(pdo_sgsrhs h);
  * runtime's random_number.  set_velocity: physical u, v (nx, ny, nz) and w
  * (nx, ny, nz+1, edges), then project_and_prep; time and step restart at 0.
  * pdo_igrid_field: 0 u, 1 v, 2 wC, 3 w, 4 uhat, 5 vhat, 6 what, 7 duidxjC,
- * 8 duidxjE, 9 urhs, 10 vrhs, 11 wrhs, 12 uE, 13 vE (device pointers of the
- * object's storage, current after each step). */
+ * 8 duidxjE, 9 urhs, 10 vrhs, 11 wrhs, 12 uE, 13 vE, 14 whatC (device
+ * pointers of the object's storage, current after each step). */
 typedef struct pdo_igrid_s *pdo_igrid;
 typedef struct {
     int nx, ny, nz;
@@ -438,6 +445,15 @@ int pdo_igrid_time(pdo_igrid h, double *tsim, long *step, double *dt);
  * object's precision), prepares the stage and sets tsim and the step count. */
 int pdo_igrid_dump_restart(pdo_igrid h, const char *dir, int rid, void *stream);
 int pdo_igrid_read_restart(pdo_igrid h, const char *dir, int rid, long tid, void *stream);
+/* SGS closure of the step (needs use_sgs with the Smagorinsky model at init):
+ * kind 0 the eddy-viscosity model with the constants of pdo_igrid_params;
+ * 1 dynamic Smagorinsky: the standard dynamic procedure (pdo_dynproc, test
+ *   filter ratio deltaRat) gives the planar coefficient at every stage (the
+ *   reference allows it with walls only; here periodic z is homogeneous);
+ * 2 the learned closure c (filter width delta, feature dratio) with the
+ *   dynamic Smagorinsky stress wherever its dissipation is negative;
+ * 3 the learned closure alone.  No reference counterpart for kinds 2, 3. */
+int pdo_igrid_set_closure(pdo_igrid h, int kind, pdo_closure c, double delta, double dratio, double deltaRat);
 int pdo_igrid_destroy(pdo_igrid h);
 
 /* ---- learned SGS closure -----------------------------------------------------
@@ -461,7 +477,49 @@ int pdo_mlp_gradients(pdo_mlp h, const float *X, const float *T, long B, float *
 int pdo_mlp_adam_step(pdo_mlp h, float lr, float b1, float b2, float eps, void *stream);
 int pdo_mlp_train_step(pdo_mlp h, const float *X, const float *T, long B, float lr, float b1, float b2, float eps,
                        float *Y, void *stream);
+/* training on any loss of the outputs: pdo_mlp_forward_train keeps the
+ * activations of a batch (B <= max_batch) and pdo_mlp_backward takes dL/dY
+ * (B, 6) of that batch, leaving the parameter gradients for
+ * pdo_mlp_adam_step / pdo_mlp_get_grads */
+int pdo_mlp_forward_train(pdo_mlp h, const float *X, float *Y, long B, void *stream);
+int pdo_mlp_backward(pdo_mlp h, const float *dY, long B, void *stream);
 int pdo_mlp_destroy(pdo_mlp h);
+
+/* ---- learned closure: invariant features and tensor-basis stress -------------
+ * No counterpart in the reference (its judge is the eddy-viscosity family of
+ * src/incompressible/sgs_models).  From a velocity-gradient field duidxj
+ * (npts, 9 components npts apart, igrid's order du_i/dx_j at 3 i + j) and its
+ * scale g = sqrt(mean |A|^2) the closure forms a = A / g, s = sym(a) - tr/3 I,
+ * r = skew(a) and
+ *   X (npts, 10): sqrt tr s^2, sqrt -tr r^2, cbrt tr s^3, cbrt tr r^2 s,
+ *     (-tr r^2 s^2)^(1/4), signed 6th root of tr r^2 s r s^2, the principal
+ *     strains of s (descending), dratio (= Delta / Delta_0);
+ *   the stress tau = Delta^2 g^2 sum_n G_n T_n with the network outputs G
+ *     (6) and Pope's basis T1 = s, T2 = s r - r s, T3 = s^2 - tr s^2/3 I,
+ *     T4 = r^2 - tr r^2/3 I, T5 = r s^2 - s^2 r, T6 = r^2 s + s r^2 -
+ *     2/3 tr(s r^2) I (frame-indifferent).
+ * pdo_closure_features: X and, unless NULL, A9 = a (npts, 9), scale from
+ *   this field.
+ * pdo_closure_stress: features, network, stress into tau with which = 0
+ *   (11 12 22 33, 4 components npts apart: igrid's cell layout), 1 (13 23:
+ *   the edge layout) or 2 (11 12 13 22 23 33); rescale = 0 reuses the scale
+ *   of the previous call; with fallback the values already in tau stay
+ *   wherever the network's dissipation -tau_ij S_ij is negative, and
+ *   pdo_closure_fallback_count reports how many points did (synchronises).
+ * pdo_closure_loss: the training loss sum_b sum_c w_c (p_c - T_c)^2 / (9 B)
+ *   of p = sum_n G_n T_n(A9) against T (B, 6), w = (1 2 2 1 2 1), and dL/dG
+ *   (B, 6); loss (host) may be NULL (no synchronisation).
+ * FP64 arithmetic per point; buffers X, G sized for max_points. */
+int pdo_closure_create(pdo_closure *h, pdo_mlp net, long max_points);
+int pdo_closure_features(pdo_closure h, int prec, const void *duidxj, long npts, double dratio, float *X, float *A9,
+                         void *stream);
+int pdo_closure_stress(pdo_closure h, int prec, const void *duidxj, long npts, double delta, double dratio, int which,
+                       int fallback, int rescale, void *tau, void *stream);
+int pdo_closure_fallback_count(pdo_closure h, long *count, void *stream);
+int pdo_closure_scale(pdo_closure h, double *rms, void *stream);
+int pdo_closure_loss(pdo_closure h, const float *G, const float *A9, const float *T, long B, float *dG, double *loss,
+                     void *stream);
+int pdo_closure_destroy(pdo_closure h);
 
 /* ---- explicit filters: gaussianstuff / lstsqstuff ------------------------
  * gaussian%init(n, periodic) (gaussian.F90:74-93) and lstsq%init

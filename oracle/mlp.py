@@ -90,20 +90,22 @@ def forward(params, X, exact=False):
     return Y, cache
 
 
-def backward(params, cache, Y, T, exact=False):
+def backward(params, cache, Y, T, exact=False, dY=None):
     """Gradients (dW_l, db_l) of the mean squared error, FP32, with the
     upstream gradients rounded to BF16 before each matrix product (as the
-    MFMA kernels consume them).  exact=True: FP64, no rounding."""
-    B = Y.shape[0]
+    MFMA kernels consume them).  exact=True: FP64, no rounding.  dY given:
+    the gradients of any loss with dL/dY = dY (Y and T unused)."""
+    B = (Y if dY is None else dY).shape[0]
     if exact:
-        dZ = 2.0 / (B * SIZES[-1]) * (Y - T)
+        dZ = 2.0 / (B * SIZES[-1]) * (Y - T) if dY is None else np.asarray(dY, np.float64)
         grads = [None] * 4
         for i in (3, 2, 1, 0):
             grads[i] = (dZ.T @ cache["H"][i], dZ.sum(axis=0))
             if i > 0:
                 dZ = (dZ @ np.asarray(params[i][0], np.float64)) * _gelu_grad64(cache["Z"][i - 1])
         return grads
-    dZ = (np.float32(2.0 / (B * SIZES[-1])) * (Y - T)).astype(np.float32)
+    dZ = (np.float32(2.0 / (B * SIZES[-1])) * (Y - T)).astype(np.float32) if dY is None else \
+        np.asarray(dY, np.float32)
     grads = [None] * 4
     for i in (3, 2, 1, 0):
         W, _ = params[i]

@@ -883,7 +883,7 @@ class igrid:
     term, optional SGS stress and HIT forcing) and project_and_prep."""
 
     FIELDS = {"u": 0, "v": 1, "wC": 2, "w": 3, "uhat": 4, "vhat": 5, "what": 6, "duidxjC": 7, "duidxjE": 8,
-              "urhs": 9, "vrhs": 10, "wrhs": 11, "uE": 12, "vE": 13}
+              "urhs": 9, "vrhs": 10, "wrhs": 11, "uE": 12, "vE": 13, "whatC": 14}
 
     def __init__(self):
         self._h = ctypes.c_void_p()
@@ -928,8 +928,8 @@ class igrid:
                  "uE": nx * ny * (nz + 1), "vE": nx * ny * (nz + 1), "duidxjC": 9 * nx * ny * nz,
                  "duidxjE": 9 * nx * ny * (nz + 1), "uhat": nxh * ny * nz, "vhat": nxh * ny * nz,
                  "what": nxh * ny * (nz + 1), "urhs": nxh * ny * nz, "vrhs": nxh * ny * nz,
-                 "wrhs": nxh * ny * (nz + 1)}
-        spectral_ = name in ("uhat", "vhat", "what", "urhs", "vrhs", "wrhs")
+                 "wrhs": nxh * ny * (nz + 1), "whatC": nxh * ny * nz}
+        spectral_ = name in ("uhat", "vhat", "what", "urhs", "vrhs", "wrhs", "whatC")
         real = torch.float64 if self.prec == "f64" else torch.float32
         cplx = torch.complex128 if self.prec == "f64" else torch.complex64
         dt = cplx if spectral_ else real
@@ -949,6 +949,15 @@ class igrid:
         return _check(lib().pdo_igrid_read_restart(self._h, str(inputdir).encode(), int(rid), int(tid),
                                                    _stream(stream)))
 
+    def set_closure(self, kind, closure=None, delta=0.0, dratio=1.0, deltaRat=2.0):
+        """kind 0 eddy viscosity (init constants), 1 dynamic Smagorinsky, 2
+        learned + dynamic fallback, 3 learned (pdo_igrid_set_closure)"""
+        kinds = {"eddy": 0, "dynamic": 1, "learned+dynamic": 2, "learned": 3}
+        k = kinds[kind] if isinstance(kind, str) else int(kind)
+        self._closure = closure  # keep the closure object alive with the grid
+        return _check(lib().pdo_igrid_set_closure(self._h, k, closure._h if closure is not None else None, delta,
+                                                  dratio, deltaRat))
+
     def time(self):
         t, s, d = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
         _check(lib().pdo_igrid_time(self._h, ctypes.byref(t), ctypes.byref(s), ctypes.byref(d)))
@@ -957,6 +966,56 @@ class igrid:
     def destroy(self):
         if self._h:
             lib().pdo_igrid_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Closure:
+    """The learned SGS closure's pointwise algebra on the device (pdo_closure_*):
+    invariant features of a velocity-gradient field, the tensor-basis stress
+    of the network's outputs, and the training loss."""
+
+    def __init__(self, net, max_points):
+        self._h = ctypes.c_void_p()
+        self.net = net
+        _check(lib().pdo_closure_create(ctypes.byref(self._h), net._h if net is not None else None, max_points))
+
+    def features(self, duidxj, npts, dratio, X, A9=None, prec=None, stream=None):
+        """X (npts, 10) and A9 (npts, 9) of duidxj (9 components npts apart)"""
+        return _check(lib().pdo_closure_features(self._h, _prec(duidxj, prec or "f32_refine"), _ptr(duidxj), npts,
+                                                 dratio, _ptr(X), _ptr(A9), _stream(stream)))
+
+    def stress(self, duidxj, npts, delta, dratio, tau, which=2, fallback=False, rescale=True, prec=None,
+               stream=None):
+        return _check(lib().pdo_closure_stress(self._h, _prec(duidxj, prec or "f32_refine"), _ptr(duidxj), npts,
+                                               delta, dratio, which, int(fallback), int(rescale), _ptr(tau),
+                                               _stream(stream)))
+
+    def fallback_count(self, stream=None):
+        c = ctypes.c_long()
+        _check(lib().pdo_closure_fallback_count(self._h, ctypes.byref(c), _stream(stream)))
+        return c.value
+
+    def scale(self, stream=None):
+        v = ctypes.c_double()
+        _check(lib().pdo_closure_scale(self._h, ctypes.byref(v), _stream(stream)))
+        return v.value
+
+    def loss(self, G, A9, T, dG, sync=True, stream=None):
+        """training loss (a float when sync, else None) and dL/dG into dG"""
+        v = ctypes.c_double()
+        _check(lib().pdo_closure_loss(self._h, _ptr(G), _ptr(A9), _ptr(T), G.shape[0], _ptr(dG),
+                                      ctypes.byref(v) if sync else None, _stream(stream)))
+        return v.value if sync else None
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_closure_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):
@@ -1078,6 +1137,18 @@ class SGSNet:
     def train_step(self, X, T, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, Y=None, stream=None):
         return _check(lib().pdo_mlp_train_step(self._h, _ptr(X), _ptr(T), X.shape[0], lr, b1, b2, eps, _ptr(Y),
                                                _stream(stream)))
+
+    def forward_train(self, X, Y, stream=None):
+        """forward pass keeping the activations for backward()"""
+        return _check(lib().pdo_mlp_forward_train(self._h, _ptr(X), _ptr(Y), X.shape[0], _stream(stream)))
+
+    def backward(self, dY, stream=None):
+        """parameter gradients of a loss with dL/dY = dY (B, 6) for the batch of
+        the last forward_train"""
+        return _check(lib().pdo_mlp_backward(self._h, _ptr(dY), dY.shape[0], _stream(stream)))
+
+    def adam_step(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, stream=None):
+        return _check(lib().pdo_mlp_adam_step(self._h, lr, b1, b2, eps, _stream(stream)))
 
     def destroy(self):
         if self._h:

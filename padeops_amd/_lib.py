@@ -114,12 +114,14 @@ SIGNATURES = {
     "pdo_sgsrhs_init": ([_pvp, _vp, _vp, _vp, _vp, _c_int], _c_int),
     "pdo_sgsrhs_compute": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "pdo_sgsrhs_fields": ([_vp, _pvp, _pvp, _pvp, _pvp], _c_int),
+    "pdo_sgsrhs_set_closure": ([_vp, _vp, _c_double, _c_double, _c_int], _c_int),
     "pdo_sgsrhs_destroy": ([_vp], _c_int),
     "pdo_igrid_init": ([_pvp, _vp, _c_int], _c_int),
     "pdo_igrid_set_velocity": ([_vp, _vp, _vp, _vp, _vp], _c_int),
     "pdo_igrid_step": ([_vp, _c_int, _vp], _c_int),
     "pdo_igrid_field": ([_vp, _c_int, _pvp], _c_int),
     "pdo_igrid_time": ([_vp, _pdbl, ctypes.POINTER(ctypes.c_long), _pdbl], _c_int),
+    "pdo_igrid_set_closure": ([_vp, _c_int, _vp, _c_double, _c_double, _c_double], _c_int),
     "pdo_igrid_destroy": ([_vp], _c_int),
     "pdo_igrid_dump_restart": ([_vp, ctypes.c_char_p, _c_int, _vp], _c_int),
     "pdo_igrid_read_restart": ([_vp, ctypes.c_char_p, _c_int, ctypes.c_long, _vp], _c_int),
@@ -133,7 +135,17 @@ SIGNATURES = {
     "pdo_mlp_adam_step": ([_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp], _c_int),
     "pdo_mlp_train_step": ([_vp, _vp, _vp, ctypes.c_long, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                             ctypes.c_float, _vp, _vp], _c_int),
+    "pdo_mlp_forward_train": ([_vp, _vp, _vp, ctypes.c_long, _vp], _c_int),
+    "pdo_mlp_backward": ([_vp, _vp, ctypes.c_long, _vp], _c_int),
     "pdo_mlp_destroy": ([_vp], _c_int),
+    "pdo_closure_create": ([_pvp, _vp, ctypes.c_long], _c_int),
+    "pdo_closure_features": ([_vp, _c_int, _vp, ctypes.c_long, _c_double, _vp, _vp, _vp], _c_int),
+    "pdo_closure_stress": ([_vp, _c_int, _vp, ctypes.c_long, _c_double, _c_double, _c_int, _c_int, _c_int, _vp, _vp],
+                           _c_int),
+    "pdo_closure_fallback_count": ([_vp, ctypes.POINTER(ctypes.c_long), _vp], _c_int),
+    "pdo_closure_scale": ([_vp, _pdbl, _vp], _c_int),
+    "pdo_closure_loss": ([_vp, _vp, _vp, _vp, ctypes.c_long, _vp, _pdbl, _vp], _c_int),
+    "pdo_closure_destroy": ([_vp], _c_int),
     "pdo_spectral_init": ([_pvp, _c_int, _c_int, _c_int, _c_double, _c_double, _c_double, _c_int, _c_double,
                            _c_int], _c_int),
     "pdo_spectral_fft": ([_vp, _c_int, _vp, _vp, _vp], _c_int),

@@ -140,6 +140,13 @@ __device__ __forceinline__ T rhs_row(const TW (&win)[W], int j, const T *c, cons
     return stencil_row<FAM, T>(win, j, c);
 }
 
+// refined modes: form the first solve's RHS in FP64 (shared with the
+// residual) for the wide stencils, in FP32 for the 5-point ones.  512^3
+// passes: cf90 x/y/z 0.246/0.266/0.270 ms with FP64, 0.273/0.288/0.300 with
+// FP32; cd06 x/y 0.207/0.222 with FP64, 0.195/0.217 with FP32 (88 instead of
+// 108 VGPRs).  Tuning choice.
+template <int FAM> constexpr bool rhs1_fp64() { return Fam<FAM>::R > 2; }
+
 // local banded solve, in place, factors fac[j*5 + {l1,l2,inv,u1,u2}]
 // Scheduling fence every SB rows: the row coefficients are uniform scalar
 // loads; without a fence the scheduler hoists all of them (hundreds of
@@ -384,17 +391,22 @@ __device__ __forceinline__ void strided_compute(const ApplyArgs &A,
     using F = Fam<FAM>;
     constexpr int K = F::K, MM = MAXM + XR;
     constexpr bool REFINE = Mode<MODE>::REFINE;
-    using TR = typename std::conditional<REFINE, double, TC>::type;  // RHS arithmetic
     const int m = MAXM, P = A.P;
     // rows of this segment
     const int ms = MAXM + (ext ? 1 : 0);
-    TR cr[5];
+    // RHS of the first solve: with refinement its error is what the FP64
+    // residual removes, so it may be formed in the solve's arithmetic (TR1 =
+    // TC, fewer live registers) or in FP64, which the compiler shares with
+    // the residual's identical stencil (no extra FP64 work, 32 more live
+    // values); see rhs1_fp64()
+    using TR1 = typename std::conditional<REFINE && rhs1_fp64<FAM>(), double, TC>::type;
+    TR1 cs[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) cr[q] = (TR)A.c[q];
+    for (int q = 0; q < 5; ++q) cs[q] = (TR1)A.c[q];
 #pragma unroll
     for (int j = 0; j < MM; ++j) {
-        if (PER) d[j] = (TC)stencil_row<FAM, TR>(win, j, cr);
-        else if (j < MAXM || ext) d[j] = (TC)rhs_row<FAM, TR, XR>(win, j, cr, A, ty, m, ext);
+        if (PER) d[j] = (TC)stencil_row<FAM, TR1>(win, j, cs);
+        else if (j < MAXM || ext) d[j] = (TC)rhs_row<FAM, TR1, XR>(win, j, cs, A, ty, m, ext);
         else d[j] = TC(0);
     }
 
@@ -431,6 +443,9 @@ __device__ __forceinline__ void strided_compute(const ApplyArgs &A,
         // the rows of a non-periodic system farther than NBMAX from its ends,
         // the others read theirs from the table
         const double *band = A.tab_d + A.o_lhs + ty * (m + A.xr) * 5;
+        double cr[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) cr[q] = A.c[q];
         TC e[MM];
 #pragma unroll
         for (int j = 0; j < MM; ++j) {
@@ -624,7 +639,6 @@ __device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename 
     using F = Fam<FAM>;
     constexpr int K = F::K, R = F::R, MM = MAXM, W = MM + 2 * R;
     constexpr bool REFINE = Mode<MODE>::REFINE;
-    using TR = typename std::conditional<REFINE, double, TC>::type;  // RHS arithmetic
     const int m = MM, P = A.P, n = A.n_sys;
     const int s = tid % P, p = tid / P;
     const bool act = p < tpa;
@@ -642,11 +656,13 @@ __device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename 
 #pragma unroll
         for (int w = 0; w < W; ++w) win[w] = w < R ? tl[w] : (w < R + MM ? tm[w - R] : tr[w - R - MM]);
     }
-    TR cr[5];
+    // RHS of the first solve (see strided_compute)
+    using TR1 = typename std::conditional<REFINE && rhs1_fp64<FAM>(), double, TC>::type;
+    TR1 cs[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) cr[q] = (TR)A.c[q];
+    for (int q = 0; q < 5; ++q) cs[q] = (TR1)A.c[q];
 #pragma unroll
-    for (int j = 0; j < MM; ++j) d[j] = (TC)rhs_row<FAM, TR>(win, j, cr, A, ty, m);
+    for (int j = 0; j < MM; ++j) d[j] = (TC)rhs_row<FAM, TR1>(win, j, cs, A, ty, m);
 
     const int NR = 2 * K * P;
     // exchange buffers hold value r of lane (p, s) at r * NT + p * P + s: the
@@ -712,6 +728,9 @@ __device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename 
                 yr[q] = rb3[q * NT + p * P + nx];
             }
         }
+        double cr[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) cr[q] = A.c[q];
         TC e[MM];
 #pragma unroll
         for (int j = 0; j < MM; ++j) {
@@ -803,8 +822,6 @@ __global__ void __launch_bounds__(256) k_xdir(ApplyArgs A, int TP, int LD)
     using TC = typename Mode<MODE>::TC;
     using F = Fam<FAM>;
     constexpr int K = F::K, R = F::R, MM = MAXM, W = MM + 2 * R;
-    constexpr bool REFINE = Mode<MODE>::REFINE;
-    using TR = typename std::conditional<REFINE, double, TC>::type;
     const int m = MM, P = A.P, n = A.n_sys;
     const int tid = threadIdx.x;
     const int s = tid % P, p = tid / P;

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/padeops_hip.h"
+#include "pdo_closure.hpp"
 #include "pdo_igrid.hpp"
 #include "pdo_ops.hpp"
 #include "pdo_poisson.hpp"
@@ -180,6 +181,12 @@ struct SgsRhsObject {
     std::vector<void *> bufs;
     void *SC = nullptr, *SE = nullptr, *nuC = nullptr, *nuE = nullptr, *tauC = nullptr, *tauE = nullptr;
     void *cb1 = nullptr, *cb2 = nullptr;
+    // learned closure (pdo_sgsrhs_set_closure): its stress replaces the
+    // eddy-viscosity stress, which stays where the closure's dissipation is
+    // negative when `fallback` is set
+    pdo_closure closure = nullptr;
+    double cl_delta = 0, cl_dratio = 1;
+    int cl_fallback = 0;
     void free_bufs()
     {
         for (void *b : bufs) (void)hipFree(b);
@@ -294,6 +301,14 @@ int pdo_sgsrhs_compute(pdo_sgsrhs h, int prec, const void *duidxjC, const void *
     }
     // tau_ij = -2 nu S_ij
     PDO_HIP(launch_tau(mode, h->nuC, h->SC, h->nuE, h->SE, h->tauC, h->tauE, (long)h->nC, (long)h->nE, st), "tau");
+    if (h->closure) {
+        // the learned stress on cells (scale from the cell gradients) and on
+        // edges (the same scale)
+        PDO_TRY(pdo_closure_stress(h->closure, prec, duidxjC, (long)h->nC, h->cl_delta, h->cl_dratio, 0,
+                                   h->cl_fallback, 1, h->tauC, stream));
+        PDO_TRY(pdo_closure_stress(h->closure, prec, duidxjE, (long)h->nE, h->cl_delta, h->cl_dratio, 1,
+                                   h->cl_fallback, 0, h->tauE, stream));
+    }
     const size_t W = prec == PDO_F64 ? 8 : 4;
     auto tc = [&](int q) { return (const void *)((const char *)h->tauC + (size_t)q * h->nC * W); };
     auto te = [&](int q) { return (const void *)((const char *)h->tauE + (size_t)q * h->nE * W); };
@@ -320,6 +335,20 @@ int pdo_sgsrhs_compute(pdo_sgsrhs h, int prec, const void *duidxjC, const void *
     }
 #undef PDO_TRY
 #undef PDO_HIP
+    return 0;
+}
+
+int pdo_sgsrhs_set_closure(pdo_sgsrhs h, pdo_closure c, double delta, double dratio, int fallback)
+{
+    if (!h) { set_error("invalid argument"); return PDO_ERR_ARG; }
+    if (c && (long)h->nE > closure_max_points(c)) {
+        set_error("sgs: the closure object holds fewer points than the edge field");
+        return PDO_ERR_ARG;
+    }
+    h->closure = c;
+    h->cl_delta = delta;
+    h->cl_dratio = dratio;
+    h->cl_fallback = fallback ? 1 : 0;
     return 0;
 }
 

@@ -35,6 +35,14 @@ struct IgridObject {
     pdo_padepoisson pp = nullptr;
     pdo_duidxj dj = nullptr;
     pdo_sgsrhs sgs = nullptr;
+    // SGS closure kind (pdo_igrid_set_closure): 0 eddy viscosity with the
+    // init constants, 1 dynamic Smagorinsky, 2 learned + dynamic fallback,
+    // 3 learned; the dynamic procedure's strain, kernel and coefficient
+    int ckind = 0;
+    pdo_dynproc dyn = nullptr;
+    pdo_spectral spD = nullptr;  // the dynamic procedure's (x-y) spectral object
+    void *dS = nullptr, *dD = nullptr;
+    double *dlam = nullptr;
     std::vector<void *> bufs;
     // sizes in real words: physical cells / edges, spectra (complex values
     // counted twice) of cells / edges
@@ -63,6 +71,8 @@ struct IgridObject {
     {
         for (void *b : bufs) (void)hipFree(b);
         if (sgs) pdo_sgsrhs_destroy(sgs);
+        if (dyn) pdo_dynproc_destroy(dyn);
+        if (spD) pdo_spectral_destroy(spD);
         if (dj) pdo_duidxj_destroy(dj);
         if (pp) pdo_padepoisson_destroy(pp);
         if (d6) pdo_pade6stagg_destroy(d6);
@@ -169,8 +179,18 @@ int populate_rhs(pdo_igrid h, void *ur, void *vr, void *wr, hipStream_t st)
         PDO_HIP(launch_visc(mode, wr, h->cur_w, h->d2w, h->spE->d_k1, h->spE->d_k2, h->nxh, h->ny, nsE, invRe, st),
                 "viscous");
     }
-    // Step 6: SGS stress divergence
-    if (h->sgs) PDO_TRY(pdo_sgsrhs_compute(h->sgs, h->prec, h->gC, h->gE, nullptr, ur, vr, wr, s));
+    // Step 6: SGS stress divergence (with the dynamic procedure's planar
+    // coefficient of this stage for closure kinds 1 and 2)
+    if (h->sgs) {
+        const double *lam = nullptr;
+        if (h->ckind == 1 || h->ckind == 2) {
+            PDO_TRY(pdo_sgs_kernel(h->prec, 0, h->gC, h->dS, h->dD, (long)nC, 0.0, 0.0, 0.0, s));
+            PDO_TRY(pdo_dynproc_compute(h->dyn, h->prec, h->u, h->v, h->wC, h->cur_u, h->cur_v, h->whC, h->dS, h->dD,
+                                        h->dlam, s));
+            lam = h->dlam;
+        }
+        PDO_TRY(pdo_sgsrhs_compute(h->sgs, h->prec, h->gC, h->gE, lam, ur, vr, wr, s));
+    }
     // Step 8: HIT forcing; new wave vectors at the first stage of a step
     if (h->p.use_forcing) {
         if (h->newTimeStep) h->draws++;
@@ -398,9 +418,34 @@ int pdo_igrid_field(pdo_igrid h, int which, const void **ptr)
 {
     if (!h || !ptr) { set_error("invalid argument"); return PDO_ERR_ARG; }
     const void *f[] = {h->u, h->v, h->wC, h->w, h->uh[0], h->vh[0], h->wh[0], h->gC, h->gE, h->ur, h->vr, h->wr,
-                       h->uE, h->vE};
+                       h->uE, h->vE, h->whC};
     if (which < 0 || which >= (int)(sizeof(f) / sizeof(f[0]))) { set_error("unknown field"); return PDO_ERR_ARG; }
     *ptr = f[which];
+    return 0;
+}
+
+int pdo_igrid_set_closure(pdo_igrid h, int kind, pdo_closure c, double delta, double dratio, double deltaRat)
+{
+    if (!h || kind < 0 || kind > 3) { set_error("invalid argument"); return PDO_ERR_ARG; }
+    if (kind > 0 && (!h->sgs || h->p.sgs.model != 0)) {
+        set_error("igrid closure: init with use_sgs and the Smagorinsky model");
+        return PDO_ERR_ARG;
+    }
+    if (kind >= 2 && !c) { set_error("igrid closure: kinds 2 and 3 need a closure object"); return PDO_ERR_ARG; }
+    if ((kind == 1 || kind == 2) && !h->dyn) {
+        // the test filter acts in x and y only (standardDynamicProcedure.F90:
+        // 165-207); InitTestFilter refuses spectral objects with periodic z
+        // (spectral.F90:1339-1342), so the procedure gets an object of this
+        // grid without the periodic-z masks: the same 2D transforms
+        int st = pdo_spectral_init(&h->spD, h->nx, h->ny, h->nz, h->dx, h->dy, h->dz, 0, 2.0 / 3.0, 0);
+        if (!st) st = pdo_dynproc_init(&h->dyn, h->spD, deltaRat);
+        if (!st) st = h->alloc(&h->dS, 6 * h->nC * h->W);
+        if (!st) st = h->alloc(&h->dD, h->nC * h->W);
+        if (!st) st = h->alloc((void **)&h->dlam, h->nz * sizeof(double));
+        if (st) return st;
+    }
+    PDO_TRY(pdo_sgsrhs_set_closure(h->sgs, kind >= 2 ? c : nullptr, delta, dratio, kind == 2));
+    h->ckind = kind;
     return 0;
 }
 

@@ -16,6 +16,7 @@ struct MlpObject {
     unsigned short *Wb = nullptr, *act = nullptr;
     MlpCache cache{};
     long t = 0;  // Adam step count
+    long cached_batch = 0;  // batch of the activations in cache (pdo_mlp_forward_train)
     std::vector<void *> allocs;
     ~MlpObject()
     {
@@ -111,7 +112,26 @@ int pdo_mlp_gradients(pdo_mlp h, const float *X, const float *T, long B, float *
     hipStream_t st = (hipStream_t)stream;
     int rc;
     if ((rc = ok(launch_mlp_forward(X, Yo, h->Wb, h->P, B, h->cache, st), "mlp forward"))) return rc;
-    return ok(launch_mlp_backward(Yo, T, h->Wb, B, h->cache, h->partial, h->grads, st), "mlp backward");
+    h->cached_batch = B;
+    return ok(launch_mlp_backward(Yo, T, nullptr, h->Wb, B, h->cache, h->partial, h->grads, st), "mlp backward");
+}
+
+int pdo_mlp_forward_train(pdo_mlp h, const float *X, float *Y, long B, void *stream)
+{
+    if (!h || !X || !Y || B <= 0 || B > h->max_batch) { set_error("invalid argument or batch too large"); return PDO_ERR_ARG; }
+    h->cached_batch = B;
+    return ok(launch_mlp_forward(X, Y, h->Wb, h->P, B, h->cache, (hipStream_t)stream), "mlp forward");
+}
+
+int pdo_mlp_backward(pdo_mlp h, const float *dY, long B, void *stream)
+{
+    if (!h || !dY || B <= 0) { set_error("invalid argument"); return PDO_ERR_ARG; }
+    if (B != h->cached_batch) {
+        set_error("pdo_mlp_backward: batch differs from the last pdo_mlp_forward_train");
+        return PDO_ERR_ARG;
+    }
+    return ok(launch_mlp_backward(nullptr, nullptr, dY, h->Wb, B, h->cache, h->partial, h->grads,
+                                  (hipStream_t)stream), "mlp backward");
 }
 
 int pdo_mlp_adam_step(pdo_mlp h, float lr, float b1, float b2, float eps, void *stream)

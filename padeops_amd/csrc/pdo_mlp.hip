@@ -134,9 +134,11 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float *__restrict__ X
 
 // ---------------------------------------------------------------------------
 // backward: per-workgroup gradient partials of the mean squared error
-// loss = mean over B x 6 of (Y - T)^2; partial[blockIdx.x][MLP_NPARAM]
+// loss = mean over B x 6 of (Y - T)^2, or of any loss whose gradient with
+// respect to the outputs is dY; partial[blockIdx.x][MLP_NPARAM]
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_mlp_backward(const float *__restrict__ Y, const float *__restrict__ T,
+                                                     const float *__restrict__ dY,
                                                      const unsigned short *__restrict__ Wb, long B,
                                                      MlpCache cache, float *__restrict__ partial)
 {
@@ -154,7 +156,10 @@ __global__ void __launch_bounds__(256) k_mlp_backward(const float *__restrict__ 
     for (int i = threadIdx.x; i < TILE * MLP_OUT_PAD; i += blockDim.x) {
         const int m = i / MLP_OUT_PAD, n = i % MLP_OUT_PAD;
         const long s = row0 + m;
-        const float d = (s < B && n < MLP_OUT) ? scale * (Y[s * MLP_OUT + n] - T[s * MLP_OUT + n]) : 0.0f;
+        // dL/dY: of the mean squared error against T, or given (dY)
+        float d = 0.0f;
+        if (s < B && n < MLP_OUT)
+            d = dY ? dY[s * MLP_OUT + n] : scale * (Y[s * MLP_OUT + n] - T[s * MLP_OUT + n]);
         sDf[m * 65 + n] = d;
         sD[m * HS + n] = to_bf16(d);
     }
@@ -251,12 +256,12 @@ hipError_t launch_mlp_forward(const float *X, float *Y, const unsigned short *Wb
     return hipGetLastError();
 }
 
-hipError_t launch_mlp_backward(const float *Y, const float *T, const unsigned short *Wb, long B,
+hipError_t launch_mlp_backward(const float *Y, const float *T, const float *dY, const unsigned short *Wb, long B,
                                const MlpCache &cache, float *partial, float *grads, hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
     const unsigned nb = nblocks(B);
-    hipLaunchKernelGGL(k_mlp_backward, dim3(nb), dim3(256), 0, st, Y, T, Wb, B, cache, partial);
+    hipLaunchKernelGGL(k_mlp_backward, dim3(nb), dim3(256), 0, st, Y, T, dY, Wb, B, cache, partial);
     hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, partial, (int)nb, grads);
     return hipGetLastError();
 }

@@ -28,7 +28,9 @@ struct MlpCache {
 
 hipError_t launch_mlp_forward(const float *X, float *Y, const unsigned short *Wb, const float *P, long B,
                               const MlpCache &cache, hipStream_t st);
-hipError_t launch_mlp_backward(const float *Y, const float *T, const unsigned short *Wb, long B,
+// dY == nullptr: gradient of the mean squared error of Y against T;
+// otherwise dY is dL/dY (B, MLP_OUT) and Y, T are unused
+hipError_t launch_mlp_backward(const float *Y, const float *T, const float *dY, const unsigned short *Wb, long B,
                                const MlpCache &cache, float *partial, float *grads, hipStream_t st);
 hipError_t launch_mlp_adam(float *P, float *M, float *V, const float *grads, unsigned short *Wb, float lr, float b1,
                            float b2, float eps, float c1, float c2, hipStream_t st);
