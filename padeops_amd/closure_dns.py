@@ -104,39 +104,50 @@ def _lowpass_sample(F, N, m, kc, zshift):
     return torch.fft.ifftn(G).real * (m ** 3 / N ** 3)
 
 
-def filter_snapshot(u, v, wE, m):
-    """Filtered fields and exact SGS stress on the m^3 LES grid from DNS u, v
-    (N^3 cells) and w (N+1 edges): (ub, vb on cells, wb on the m+1 edges,
-    tau (6, m, m, m) deviatoric 11 12 13 22 23 33 on cells), torch (z, y, x)."""
+def filter_snapshot(u, v, wE, grids):
+    """Filtered fields and exact SGS stress on each m^3 LES grid of `grids`
+    (an int or a sequence) from DNS u, v (N^3 cells) and w (N+1 edges): per
+    grid (ub, vb on cells, wb on the m+1 edges, tau (6, m, m, m) deviatoric
+    11 12 13 22 23 33 on cells), torch (z, y, x).  The DNS transforms are
+    taken once for all grids."""
     torch = _torch()
+    single = isinstance(grids, int)
+    grids = (grids,) if single else tuple(grids)
     N = u.shape[0]
-    kc = kc_of(m)
-    r = N // m
-    assert r * m == N and kc < m // 2
-    c_shift = 0.5 * (r - 1)    # LES cell centres above the DNS cell centres
-    Fu, Fv = torch.fft.fftn(u), torch.fft.fftn(v)
-    Fw = torch.fft.fftn(wE[:N])
+    for m in grids:
+        assert (N // m) * m == N and kc_of(m) < m // 2
     kz = torch.fft.fftfreq(N, d=1.0 / N, device=u.device, dtype=torch.float64).view(-1, 1, 1)
+    Fw = torch.fft.fftn(wE[:N])
     # w on the DNS cells (half a point up), for the products
     wc = torch.fft.ifftn(Fw * torch.exp(1j * (2 * math.pi / N) * kz * 0.5)).real
-    ub = _lowpass_sample(Fu, N, m, kc, c_shift)
-    vb = _lowpass_sample(Fv, N, m, kc, c_shift)
-    wbc = _lowpass_sample(Fw, N, m, kc, 0.5 * r)       # w at the LES cells
-    wbe = _lowpass_sample(Fw, N, m, kc, 0.0)           # w at the LES edges
-    del Fu, Fv, Fw
+    out = {m: {} for m in grids}
+
+    def cut(F, name, shift_cells):
+        # shift_cells(r): the LES grid's z origin above the DNS grid's, in
+        # DNS points, for r = N / m
+        for m in grids:
+            out[m][name] = _lowpass_sample(F, N, m, kc_of(m), shift_cells(N // m))
+
+    cell = lambda r: 0.5 * (r - 1)  # noqa: E731  LES cell centres above DNS cell centres
+    cut(Fw, "wbc", lambda r: 0.5 * r)
+    cut(Fw, "wbe", lambda r: 0.0)
+    del Fw
     vel = (u, v, wc)
-    velb = (ub, vb, wbc)
-    tau = []
-    for i, j in IDX6:
-        Fp = torch.fft.fftn(vel[i] * vel[j])
-        tau.append(_lowpass_sample(Fp, N, m, kc, c_shift) - velb[i] * velb[j])
-        del Fp
-    tau = torch.stack(tau)
-    trace = (tau[0] + tau[3] + tau[5]) / 3.0
-    for q in (0, 3, 5):
-        tau[q] -= trace
-    wb = torch.cat([wbe, wbe[:1]], 0)
-    return ub, vb, wb, tau
+    for q, c in enumerate((u, v)):
+        cut(torch.fft.fftn(c), "u%d" % q, cell)
+    for q, (i, j) in enumerate(IDX6):
+        cut(torch.fft.fftn(vel[i] * vel[j]), "p%d" % q, cell)
+    res = []
+    for m in grids:
+        o = out[m]
+        velb = (o["u0"], o["u1"], o["wbc"])
+        tau = torch.stack([o["p%d" % q] - velb[i] * velb[j] for q, (i, j) in enumerate(IDX6)])
+        trace = (tau[0] + tau[3] + tau[5]) / 3.0
+        for q in (0, 3, 5):
+            tau[q] -= trace
+        wb = torch.cat([o["wbe"], o["wbe"][:1]], 0)
+        res.append((o["u0"], o["u1"], wb, tau))
+    return res[0] if single else res
 
 
 def spectrum(u, v, w):
@@ -196,18 +207,29 @@ class LesFeatures:
 # ---------------------------------------------------------------------------
 # 4. training
 # ---------------------------------------------------------------------------
+# offsets of the output layer in the network's parameter vector (W4 16 x 64, b4 16)
+_W4, _B4 = 9408, 10432
+
+
 def train(net, closure, X, A9, T, steps, batch=65536, lr=1e-3, seed=0, log=None):
-    """Adam on random batches; returns the loss history (every 50 steps)."""
+    """Adam on random batches of targets scaled to unit rms (sigma: the
+    weighted rms of T); the output layer is multiplied by sigma at the end, so
+    the trained network gives the unscaled coefficients.  Returns the history
+    of the scaled loss (every 50 steps: the fraction of the targets' mean
+    square left unexplained) and sigma."""
     torch = _torch()
     gen = torch.Generator(device=X.device)
     gen.manual_seed(seed)
     N = X.shape[0]
+    w = torch.tensor([1, 2, 2, 1, 2, 1], dtype=torch.float64, device=X.device)
+    sigma = float(torch.sqrt((T.double() ** 2 * w).sum() / (9.0 * N)))
     G = torch.empty((batch, 6), dtype=torch.float32, device=X.device)
     dG = torch.empty_like(G)
     hist = []
     for it in range(steps):
         idx = torch.randint(0, N, (batch,), generator=gen, device=X.device)
-        xb, ab, tb = X[idx].contiguous(), A9[idx].contiguous(), T[idx].contiguous()
+        xb, ab = X[idx].contiguous(), A9[idx].contiguous()
+        tb = (T[idx] / sigma).contiguous()
         assert net.forward_train(xb, G) == 0
         sync = it % 50 == 0 or it == steps - 1
         lv = closure.loss(G, ab, tb, dG, sync=sync)
@@ -218,8 +240,11 @@ def train(net, closure, X, A9, T, steps, batch=65536, lr=1e-3, seed=0, log=None)
         if sync:
             hist.append((it, lv))
             if log:
-                log("train step %d loss %.5f" % (it, lv))
-    return hist
+                log("train step %d scaled loss %.5f" % (it, lv))
+    p = net.get_params()
+    p[_W4:_B4 + 16] *= sigma
+    net.set_params(p)
+    return hist, sigma
 
 
 # ---------------------------------------------------------------------------
@@ -402,8 +427,7 @@ def run(N=256, les_grids=(64, 128), Re=600.0, nsnap=200, every=10, spinup_time=6
         u, v, w = dns.fields()
         urms.append(float(torch.sqrt((u * u + v * v + w[:N] * w[:N]).mean() / 3)))
         held = snap >= nsnap - ntest
-        for m in les_grids:
-            ub, vb, wb, tau = filter_snapshot(u, v, w, m)
+        for m, (ub, vb, wb, tau) in zip(les_grids, filter_snapshot(u, v, w, les_grids)):
             Eacc[m] = Eacc[m] + spectrum(ub, vb, wb[:m])
             if held:
                 if snap in keep_at:
@@ -425,8 +449,9 @@ def run(N=256, les_grids=(64, 128), Re=600.0, nsnap=200, every=10, spinup_time=6
     del Xs, As, Ts
     res["train_points"] = int(X.shape[0])
     log("training on %d points (%s)" % (X.shape[0], clock()))
-    hist = train(net, closure, X, A9, T, train_steps, log=log)
-    res["train_loss"] = hist
+    hist, sigma = train(net, closure, X, A9, T, train_steps, log=log)
+    res["train_loss_scaled"] = hist
+    res["target_rms"] = sigma
     del X, A9, T
     res["apriori"] = {}
     for m in les_grids:

@@ -30,5 +30,5 @@ def test_closure_pipeline_small(cuda):
     post = res["aposteriori"]
     assert post["learned+dynamic"]["finite"]
     assert post["dynamic"]["finite"]
-    hist = res["train_loss"]
+    hist = res["train_loss_scaled"]
     assert hist[-1][1] < hist[0][1]
