@@ -141,11 +141,14 @@ __device__ __forceinline__ T rhs_row(const TW (&win)[W], int j, const T *c, cons
 }
 
 // refined modes: form the first solve's RHS in FP64 (shared with the
-// residual) for the wide stencils, in FP32 for the 5-point ones.  512^3
-// passes: cf90 x/y/z 0.246/0.266/0.270 ms with FP64, 0.273/0.288/0.300 with
-// FP32; cd06 x/y 0.207/0.222 with FP64, 0.195/0.217 with FP32 (88 instead of
-// 108 VGPRs).  Tuning choice.
-template <int FAM> constexpr bool rhs1_fp64() { return Fam<FAM>::R > 2; }
+// residual) for the wide stencils of periodic systems, in FP32 otherwise.
+// 512^3 periodic passes: cf90 x/y/z 0.246/0.266/0.270 ms with FP64,
+// 0.273/0.288/0.300 with FP32; cd06 x/y 0.207/0.222 with FP64, 0.195/0.217
+// with FP32 (88 instead of 108 VGPRs).  Non-periodic systems (closure rows,
+// more live registers) gain most from FP32: the 512 x 512 x 256 wall-normal
+// cd06stagg passes went from 0.43 / 0.37 ms to 0.23 / 0.13 ms.  Tuning
+// choice.
+template <int FAM, bool PER> constexpr bool rhs1_fp64() { return PER && Fam<FAM>::R > 2; }
 
 // local banded solve, in place, factors fac[j*5 + {l1,l2,inv,u1,u2}]
 // Scheduling fence every SB rows: the row coefficients are uniform scalar
@@ -399,7 +402,7 @@ __device__ __forceinline__ void strided_compute(const ApplyArgs &A,
     // TC, fewer live registers) or in FP64, which the compiler shares with
     // the residual's identical stencil (no extra FP64 work, 32 more live
     // values); see rhs1_fp64()
-    using TR1 = typename std::conditional<REFINE && rhs1_fp64<FAM>(), double, TC>::type;
+    using TR1 = typename std::conditional<REFINE && rhs1_fp64<FAM, PER>(), double, TC>::type;
     TR1 cs[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) cs[q] = (TR1)A.c[q];
@@ -657,7 +660,7 @@ __device__ __forceinline__ void xdir_compute(const ApplyArgs &A, const typename 
         for (int w = 0; w < W; ++w) win[w] = w < R ? tl[w] : (w < R + MM ? tm[w - R] : tr[w - R - MM]);
     }
     // RHS of the first solve (see strided_compute)
-    using TR1 = typename std::conditional<REFINE && rhs1_fp64<FAM>(), double, TC>::type;
+    using TR1 = typename std::conditional<REFINE && rhs1_fp64<FAM, PER>(), double, TC>::type;
     TR1 cs[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) cs[q] = (TR1)A.c[q];
