@@ -298,13 +298,42 @@ def dynamic_smagorinsky_stress(les, deltaRat=2.0):
     return tau, lam
 
 
+AMD_C = 1.67   # the AMD model constant of the reference's HIT input (Csgs, SGSModelID = 2)
+
+
+def amd_constants(m, C=AMD_C):
+    """camd_x, camd_y, camd_z of init_amd (AMD.F90:1-12) on the m^3 grid of
+    side 2 pi: C dx sqrt(1/12) in every direction (the z direction takes the
+    same factor here, periodic z having no wall-normal Poincare constant)"""
+    dx = L / m
+    c = C * dx * math.sqrt(1.0 / 12.0)
+    return (c, c, c)
+
+
+def amd_stress(les, C=AMD_C):
+    """tau = -2 nu_AMD S_ij (AMD kernel with the reference's constants) on the
+    field set in the LesFeatures grid; (6, n) deviatoric"""
+    torch = _torch()
+    n = les.m ** 3
+    A = les.g.field("duidxjC")
+    S = torch.empty(6 * n, dtype=torch.float64, device=A.device)
+    nu = torch.empty(n, dtype=torch.float64, device=A.device)
+    assert P.sgs_kernel(A, S, nu, n, model="amd", camd=amd_constants(les.m, C), prec="f64") == 0
+    tau = (-2.0 * nu.view(1, n)) * S.view(6, n)
+    trace = (tau[0] + tau[3] + tau[5]) / 3.0
+    tau = tau.clone()
+    for q in (0, 3, 5):
+        tau[q] -= trace
+    return tau
+
+
 def apriori(les, net, closure, snaps, deltaRat=2.0):
     """scores of the network and the dynamic Smagorinsky stress on held-out
     filtered snapshots [(ub, vb, wb, tau)]"""
     torch = _torch()
     n = les.m ** 3
-    out = {"mlp_corr": [], "mlp_err": [], "dyn_corr": [], "dyn_err": [], "diss_dns": [], "diss_mlp": [],
-           "diss_dyn": [], "mlp_backscatter_fraction": []}
+    out = {"mlp_corr": [], "mlp_err": [], "dyn_corr": [], "dyn_err": [], "amd_corr": [], "amd_err": [],
+           "diss_dns": [], "diss_mlp": [], "diss_dyn": [], "diss_amd": [], "mlp_backscatter_fraction": []}
     for ub, vb, wb, tau in snaps:
         A = les.set(ub, vb, wb)
         t6 = tau.reshape(6, n)
@@ -312,8 +341,13 @@ def apriori(les, net, closure, snaps, deltaRat=2.0):
         assert closure.stress(A, n, les.delta, les.dratio, pred, which=2, prec="f64") == 0
         pred = pred.view(6, n)
         tdyn, _ = dynamic_smagorinsky_stress(les, deltaRat)
+        tamd = amd_stress(les)
         c, e = stress_scores(pred, t6)
         cd, ed = stress_scores(tdyn, t6)
+        ca, ea = stress_scores(tamd, t6)
+        out["amd_corr"].append(ca)
+        out["amd_err"].append(ea)
+        out["diss_amd"].append(dissipation(tamd, A))
         out["mlp_corr"].append(c)
         out["mlp_err"].append(e)
         out["dyn_corr"].append(cd)
@@ -332,16 +366,19 @@ def apriori(les, net, closure, snaps, deltaRat=2.0):
 # ---------------------------------------------------------------------------
 def les_run(m, Re, forcing, ub, vb, wb, kind, closure=None, d0=None, T=4.0, sample_every=10, CFL=1.0,
             prec="f32_refine", deltaRat=2.0, log=None):
-    """m^3 LES from (ub, vb, wb); kind None (no model), "dynamic",
-    "learned+dynamic" or "learned".  Returns the time-averaged spectrum over
+    """m^3 LES from (ub, vb, wb); kind None (no model), "amd" (the AMD model
+    with the reference's constant), "dynamic", "learned+dynamic" or "learned".  Returns the time-averaged spectrum over
     the second half of [0, T], the final time and steps, and whether the run
     stayed finite."""
     torch = _torch()
     g = P.igrid()
-    sgs = None if kind is None else dict(model="smagorinsky", cglobal=0.0)
+    if kind == "amd":
+        sgs = dict(model="amd", camd=amd_constants(m), cglobal=1.0)
+    else:
+        sgs = None if kind is None else dict(model="smagorinsky", cglobal=0.0)
     assert g.init(m, m, m, L, L, L, Re=Re, scheme=2, CFL=CFL, sgs=sgs, forcing=forcing, prec=prec) == 0
     dt = torch.float64 if prec == "f64" else torch.float32
-    if kind is not None:
+    if kind not in (None, "amd"):
         delta = delta_of(m)
         g.set_closure(kind, closure, delta=delta, dratio=delta / (d0 or delta), deltaRat=deltaRat)
     assert g.set_velocity(ub.reshape(-1).to(dt), vb.reshape(-1).to(dt), wb.reshape(-1).to(dt)) == 0
@@ -463,7 +500,7 @@ def run(N=256, les_grids=(64, 128), Re=600.0, nsnap=200, every=10, spinup_time=6
     Eref = Eacc[m] / nsnap
     ub, vb, wb, _ = test[m][0]
     post = {"Eref": Eref.tolist(), "kc": kc_of(m)}
-    for kind in (None, "dynamic", "learned+dynamic"):
+    for kind in (None, "amd", "dynamic", "learned+dynamic", "learned"):
         r = les_run(m, Re, forcing, ub, vb, wb, kind, closure=closure, d0=d0, T=les_T, log=log)
         name = kind or "no_model"
         entry = {"t": r["t"], "steps": r["steps"], "finite": r["finite"]}
