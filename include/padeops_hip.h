@@ -285,9 +285,49 @@ int pdo_spectral_destroy(pdo_spectral h);
  *   DivergenceCheck :1165-1244 (steps 1-3): the real divergence field
  *       (nx, ny, nz) of the spectral velocity. */
 int pdo_padepoisson_init_periodic(pdo_padepoisson *h, pdo_spectral sp, double dz);
+/* the non-periodic branch (walls in z, PeriodicInZ = .false., without the
+ * Stokes pressure): init :181-232, PressureProjection :434-624, the
+ * reference's algorithm.  f = i (k1 u + k2 v) on the cells; the even
+ * extension of f and the odd extension of w (2 nz points each) are
+ * transformed in z; P = -(F + i kcm W) / |k|^2 (|k|^2 <= 1e-14: 0) and
+ * W -= i kcp P, with kcm, kcp = k3' exp(-+ i k3 dz / 2) and k3' the
+ * cd06stagg modified wavenumber of GetWaveNums(2 nz, dz); both go back,
+ * u -= i k1 p, v -= i k2 p, and w = 0 on both walls.  The odd extension
+ * assumes w = 0 on the walls on input (no penetration); w given there is
+ * extended as it is, as in the reference, and the result is then not
+ * divergence-free next to that wall.  sp must not be periodic in z; 2 nz a
+ * power of two in [8,1024].  pdo_padepoisson_divergence then uses the
+ * odd-ended E2C (the reference's derZ, :150-151). */
+int pdo_padepoisson_init_nonperiodic(pdo_padepoisson *h, pdo_spectral sp, double dz);
+/* computeStokesPressure = .true. with Lz (init :236-307) on an object of
+ * pdo_padepoisson_init_nonperiodic.  Step 0 of the projection and of both
+ * pressures (ProjectStokesPressure :320-384, GetStokesPressure :641-714)
+ * then removes w on both walls line by line with the potential
+ * phi = chat_bot cosh(lam (Lz - z)) + chat_top cosh(lam z), lam = |(k1, k2)|,
+ * chat_bot = -w(bottom) denFact, chat_top = w(top) denFact,
+ * denFact = 1 / (lam sinh(lam Lz) + 1e-13) (0 where lam Lz >= 500, below
+ * 1e-16 and on the mean line); u -= i k1 phi, v -= i k2 phi,
+ * w -= dphi/dz, with the reference's caps of cosh and sinh from an argument
+ * of 32 on.  From Step 0 on, k1 and k2 are the plain GetWaveNums values, not
+ * the spectral object's (Nyquist modes of the other sign), as in the
+ * reference.  The tables are evaluated on the fly; one call per object. */
+int pdo_padepoisson_set_stokes(pdo_padepoisson h, double Lz);
 int pdo_padepoisson_projection(pdo_padepoisson h, int prec, void *uhat, void *vhat, void *what, void *stream);
 int pdo_padepoisson_divergence(pdo_padepoisson h, int prec, const void *uhat, const void *vhat, const void *what,
                                void *divergence, void *stream);
+/* getPressure (update 0; :752-898, periodic :716-750) and
+ * getPressureAndUpdateRHS (update 1; :952-1163, periodic :900-949):
+ * pressure (nx, ny, nz) real, of prec's storage type, the inverse 2D
+ * transform of the pressure p of the projection.  update 0 leaves uhat,
+ * vhat, what as they are (with walls and the Stokes pressure, its step runs
+ * on copies held by the object); update 1 projects them as
+ * pdo_padepoisson_projection does.  With the Stokes pressure the pressure
+ * adds i phi (phat_z1 + phat_z2 of the reference, :882-890): the reference's
+ * factor, although the velocity takes -grad phi.  getPressureAndUpdateRHS
+ * adds the i phi of the same call; the reference adds phat_z1 + phat_z2 as
+ * its last getPressure stored them, since ProjectStokesPressure stores none. */
+int pdo_padepoisson_pressure(pdo_padepoisson h, int prec, void *uhat, void *vhat, void *what, void *pressure,
+                             int update, void *stream);
 int pdo_padepoisson_destroy(pdo_padepoisson h);
 
 /* ---- igrid%compute_duidxj --------------------------------------------------

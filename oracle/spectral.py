@@ -138,6 +138,158 @@ class PadePoissonPeriodic:
         """DivergenceCheck steps 1-3: the real divergence field."""
         return self.sp.ifft(self._div(uhat, vhat, what))
 
+    def _pressure_hat(self, uhat, vhat, what):
+        f = self._div(uhat, vhat, what)
+        return np.fft.ifft(-self.kradsq_inv * np.fft.fft(f, axis=2), axis=2)
+
+    def get_pressure(self, uhat, vhat, what):
+        """Periodic_getPressure (:716-750): the real pressure field."""
+        return self.sp.ifft(self._pressure_hat(uhat, vhat, what))
+
+    def get_pressure_and_update(self, uhat, vhat, what):
+        """Periodic_getPressureAndUpdateRHS (:900-949): the projected
+        (u, v, w) and the pressure."""
+        return self.projection(uhat, vhat, what) + (self.get_pressure(uhat, vhat, what),)
+
+
+class PadePoissonWalls:
+    """padepoisson with walls in z (PeriodicInZ = .false.), literally as the
+    reference computes it: init PadePoisson.F90:181-307, PressureProjection
+    :434-624 (steps 0-9), getPressure :752-898, getPressureAndUpdateRHS
+    :952-1163: the 2 nz-point even extension of f = i (k1 u + k2 v) and odd
+    extension of w, their z transforms, the pressure
+    -(F + i k3' e^{-i k3 dz/2} W) / |k|^2 (|k|^2 <= 1e-14 dropped),
+    W -= i k3' e^{+i k3 dz/2} P, the inverse transforms (1 / (2 nz)), and
+    w = 0 on both walls; u -= i k1 p, v -= i k2 p.
+
+    Lz given: computeStokesPressure.  Step 0 (ProjectStokesPressure /
+    GetStokesPressure, :320-384, :641-714) first removes, line by line, the
+    wall-normal velocity on both walls with the potential
+    phi = chat_bot cosh(lam (Lz - z)) + chat_top cosh(lam z), lam = |(k1, k2)|,
+    and from there on k1, k2 are the plain GetWaveNums values (k1inZ, k2inZ),
+    not the spectral object's (whose Nyquist modes carry the other sign).  The
+    pressure adds phat_z1 + phat_z2 = i phi (:882-890, :1142-1150)."""
+
+    def __init__(self, sp, dx, dy, dz, Lz=None):
+        nx, ny, nz = sp.shape
+        self.sp, self.dz, self.nz = sp, dz, nz
+        k1 = getwavenums(nx, dx)[: sp.nxh]
+        k2 = getwavenums(ny, dy)
+        k3 = getwavenums(2 * nz, dz)
+        k3m = modk(2, 2 * nz, dz)
+        self.k3cm = k3m * np.exp(-0.5j * dz * k3)
+        self.k3cp = k3m * np.exp(0.5j * dz * k3)
+        ks = k1[:, None, None] ** 2 + k2[None, :, None] ** 2 + k3m[None, None, :] ** 2
+        self.kradsq_inv = np.where(ks <= 1e-14, 0.0, 1.0 / np.where(ks <= 1e-14, 1.0, ks))
+        self.k1in, self.k2in = k1, k2
+        self.stokes = Lz is not None
+        if self.stokes:
+            self._stokes_tables(float(Lz))
+
+    def _stokes_tables(self, Lz):
+        """init :236-307 (denFact, cosh_bot/top on the cells, sinh_bot/top on
+        the edges, with the reference's caps)."""
+        nz = self.nz
+        lam = np.sqrt(self.k1in[:, None] ** 2 + self.k2in[None, :] ** 2)
+        t = lam * Lz
+        den = np.where(t < 500.0, 1.0 / (lam * np.sinh(np.minimum(t, 500.0)) + 1e-13), 0.0)
+        den = np.where(den < 1e-16, 0.0, den)
+        den[0, 0] = 0.0
+        # linspace (gridtools.F90:51-69): running sums of the step
+        step = (Lz - 0.0) / (float(nz + 1) - 1.0)
+        ze = np.zeros(nz + 1)
+        for i in range(1, nz + 1):
+            ze[i] = ze[i - 1] + step
+        zc = 0.5 * (ze[:nz] + ze[1:])
+        L3 = lam[:, :, None]
+
+        def capped(t, f, cap):
+            return np.where(t < 32.0, f(np.minimum(t, 32.0)), cap)
+
+        self.den = den
+        self.cosh_bot = capped(L3 * (Lz - zc), np.cosh, 4.0e13)
+        self.cosh_top = capped(L3 * zc, np.cosh, 1.0e13)
+        tb, tt = L3 * (Lz - ze), L3 * ze
+        self.sinh_bot = np.where(tb < 32.0, -L3 * np.sinh(np.minimum(tb, 32.0)), -4.0e13)
+        self.sinh_top = np.where(tt < 32.0, L3 * np.sinh(np.minimum(tt, 32.0)), 4.0e13)
+        self.ze, self.zc, self.Lz = ze, zc, Lz
+
+    def stokes_step(self, uhat, vhat, what):
+        """ProjectStokesPressure / GetStokesPressure: the corrected copies
+        (u, v, w) and phat_z1, phat_z2."""
+        nz = self.nz
+        k1 = self.k1in[:, None, None]
+        k2 = self.k2in[None, :, None]
+        u = np.array(uhat, dtype=np.complex128)
+        v = np.array(vhat, dtype=np.complex128)
+        w = np.array(what, dtype=np.complex128)
+        cb = -w[:, :, 0] * self.den
+        p1 = 1j * cb[:, :, None] * self.cosh_bot
+        u = u - k1 * p1
+        v = v - k2 * p1
+        w[:, :, 0] = 0
+        w[:, :, 1:] = w[:, :, 1:] - cb[:, :, None] * self.sinh_bot[:, :, 1:]
+        ct = w[:, :, nz] * self.den
+        p2 = 1j * ct[:, :, None] * self.cosh_top
+        u = u - k1 * p2
+        v = v - k2 * p2
+        w[:, :, :nz] = w[:, :, :nz] - ct[:, :, None] * self.sinh_top[:, :, :nz]
+        w[:, :, nz] = 0
+        return u, v, w, p1, p2
+
+    def _prep(self, uhat, vhat, what):
+        sp = self.sp
+        if self.stokes:
+            u, v, w, p1, p2 = self.stokes_step(uhat, vhat, what)
+            k1, k2, ps = self.k1in, self.k2in, p1 + p2
+        else:
+            u, v, w = np.asarray(uhat), np.asarray(vhat), np.asarray(what)
+            k1, k2, ps = sp.k1, sp.k2, None
+        return u, v, w, k1[:, None, None], k2[None, :, None], ps
+
+    def _solve(self, f, w, update):
+        """steps 3-7: the pressure on the cells and (update) the new w."""
+        nz = self.nz
+        fext = np.concatenate([f[:, :, ::-1], f], axis=2)
+        wext = np.concatenate([-w[:, :, nz - 1:0:-1], w], axis=2)
+        F = np.fft.fft(fext, axis=2)
+        Wt = np.fft.fft(wext, axis=2)
+        F = -(F + 1j * self.k3cm * Wt) * self.kradsq_inv
+        p = np.fft.ifft(F, axis=2)[:, :, nz:]
+        if not update:
+            return p, None
+        Wt = Wt - 1j * self.k3cp * F
+        wn = np.fft.ifft(Wt, axis=2)[:, :, nz - 1:].copy()
+        wn[:, :, 0] = 0
+        wn[:, :, nz] = 0
+        return p, wn
+
+    def _project(self, uhat, vhat, what):
+        u, v, w, k1, k2, ps = self._prep(uhat, vhat, what)
+        p, wn = self._solve(1j * (k1 * u + k2 * v), w, True)
+        return u - 1j * k1 * p, v - 1j * k2 * p, wn, p, ps
+
+    def projection(self, uhat, vhat, what):
+        return self._project(uhat, vhat, what)[:3]
+
+    def get_pressure(self, uhat, vhat, what):
+        """getPressure: the real pressure field; the inputs stay as they are."""
+        u, v, w, k1, k2, ps = self._prep(uhat, vhat, what)
+        p, _ = self._solve(1j * (k1 * u + k2 * v), w, False)
+        if ps is not None:
+            p = p + ps
+        return self.sp.ifft(p)
+
+    def get_pressure_and_update(self, uhat, vhat, what):
+        """getPressureAndUpdateRHS: the projected (u, v, w) and the pressure.
+        With the Stokes pressure, the i phi of this call is added (the
+        reference adds phat_z1 + phat_z2 as its last getPressure left them:
+        ProjectStokesPressure does not store them)."""
+        u, v, w, p, ps = self._project(uhat, vhat, what)
+        if ps is not None:
+            p = p + ps
+        return u, v, w, self.sp.ifft(p)
+
 
 # Pade6stagg (PadeDerOps.F90:57-1008), scheme cd06: the boundary object of
 # each (bot, top) in {-1 odd, 0 one-sided, 1 even}; the object derXY is

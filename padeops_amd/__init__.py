@@ -667,15 +667,25 @@ class spectral:
 
 
 class padepoisson:
-    """Periodic-in-z type(padepoisson) of src/incompressible/PadePoisson.F90 with
-    cd06 staggered z derivatives (Pade6stagg scheme cd06)."""
+    """type(padepoisson) of src/incompressible/PadePoisson.F90 with cd06
+    staggered z derivatives (Pade6stagg scheme cd06): periodic in z, or with
+    walls (PeriodicInZ = False), then optionally with the Stokes pressure
+    (computeStokesPressure = True and Lz; ignored when periodic, as in the
+    reference)."""
 
     def __init__(self):
         self._h = ctypes.c_void_p()
 
-    def init(self, sp, dz):
+    def init(self, sp, dz, PeriodicInZ=True, computeStokesPressure=False, Lz=None):
         self.sp = sp
-        return _check(lib().pdo_padepoisson_init_periodic(ctypes.byref(self._h), sp._h, dz))
+        if PeriodicInZ:
+            return _check(lib().pdo_padepoisson_init_periodic(ctypes.byref(self._h), sp._h, dz))
+        _check(lib().pdo_padepoisson_init_nonperiodic(ctypes.byref(self._h), sp._h, dz))
+        if computeStokesPressure:
+            if Lz is None:
+                raise ValueError("the Stokes pressure needs Lz")
+            _check(lib().pdo_padepoisson_set_stokes(self._h, float(Lz)))
+        return 0
 
     def _sizes(self, uh, vh, wh):
         n = self.sp._nspec()
@@ -687,6 +697,22 @@ class padepoisson:
         self._sizes(uhat, vhat, what)
         return _check(lib().pdo_padepoisson_projection(self._h, _prec(uhat, prec), _ptr(uhat), _ptr(vhat),
                                                        _ptr(what), _stream(stream)))
+
+    def _pressure(self, uhat, vhat, what, pressure, update, prec, stream):
+        self._sizes(uhat, vhat, what)
+        nx, ny, nz = self.sp.shape
+        if pressure.numel() < nx * ny * nz:
+            raise ValueError("buffer smaller than the declared shape")
+        return _check(lib().pdo_padepoisson_pressure(self._h, _prec(uhat, prec), _ptr(uhat), _ptr(vhat), _ptr(what),
+                                                     _ptr(pressure), update, _stream(stream)))
+
+    def getPressure(self, uhat, vhat, what, pressure, prec="f32_refine", stream=None):
+        """the real pressure (nx, ny, nz); the velocities stay as they are"""
+        return self._pressure(uhat, vhat, what, pressure, 0, prec, stream)
+
+    def getPressureAndUpdateRHS(self, uhat, vhat, what, pressure, prec="f32_refine", stream=None):
+        """PressureProjection of (uhat, vhat, what) and its real pressure"""
+        return self._pressure(uhat, vhat, what, pressure, 1, prec, stream)
 
     def DivergenceCheck(self, uhat, vhat, what, divergence, prec="f32_refine", stream=None):
         self._sizes(uhat, vhat, what)

@@ -157,8 +157,11 @@ SIGNATURES = {
     "pdo_spectral_set_factors": ([_vp, _pdbl, _pdbl, _pdbl], _c_int),
     "pdo_spectral_destroy": ([_vp], _c_int),
     "pdo_padepoisson_init_periodic": ([_pvp, _vp, _c_double], _c_int),
+    "pdo_padepoisson_init_nonperiodic": ([_pvp, _vp, _c_double], _c_int),
     "pdo_padepoisson_projection": ([_vp, _c_int, _vp, _vp, _vp, _vp], _c_int),
     "pdo_padepoisson_divergence": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "pdo_padepoisson_set_stokes": ([_vp, _c_double], _c_int),
+    "pdo_padepoisson_pressure": ([_vp, _c_int, _vp, _vp, _vp, _vp, _c_int, _vp], _c_int),
     "pdo_padepoisson_destroy": ([_vp], _c_int),
     "pdo_rhs_init": ([_pvp, _c_int, _c_double, _c_int], _c_int),
     "pdo_rhs_eval": ([_vp] + [_vp] * 7 + [_vp], _c_int),

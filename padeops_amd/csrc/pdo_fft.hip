@@ -896,6 +896,155 @@ static hipError_t run_zmul_t(const PoissonPlanDev &pl, const TS *in, TS *out, co
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// PadePoisson with walls in z (PadePoisson.F90:472-603), one workgroup per TX
+// (kx) lines of a ky: the 2 nz-point even extension of f (cells, nz points:
+// nz-1 .. 0, 0 .. nz-1) and odd extension of w (edges, nz+1 points:
+// -w[nz-1] .. -w[1], w[0] .. w[nz]) are transformed in LDS, then
+//   P = -(F + i kcm W) / |k|^2   (|k|^2 = ax[kx] + ay[ky] + az[kz], <= 1e-14: 0)
+//   W = W - i kcp P
+// with kcm, kcp = k3' exp(-+ i k3 dz / 2), both are transformed back (scale
+// 1 / L), f takes the second half of P and w points nz-1 .. L-1 of W, zero on
+// the two walls.  f and w are updated in place.  UPD false (getPressure,
+// :834-871): only P goes back to f, w is read only.
+// ---------------------------------------------------------------------------
+template <typename TS, typename TC, int L, int TX, bool UPD>
+__global__ void __launch_bounds__(1024) k_pp_walls(TS *f, TS *w, long na, long sb, int se, const TC *__restrict__ twL,
+                                                  const double *__restrict__ ax, const double *__restrict__ ay,
+                                                  const double *__restrict__ az, const double *__restrict__ kcm,
+                                                  const double *__restrict__ kcp)
+{
+    extern __shared__ unsigned char smem[];
+    constexpr int H = L / 2;
+    C2<TC> *tF = (C2<TC> *)smem;
+    C2<TC> *tW = tF + TX * L;
+    C2<TC> *twl = tW + TX * L;
+    const int ln = threadIdx.x & (TX - 1), jb = threadIdx.x / TX;
+    const long a = (long)blockIdx.x * TX + ln;
+    const bool act = a < na;
+    const long off = 2 * ((act ? a : 0) + (long)blockIdx.y * sb);
+    TS *const lf = f + off;
+    TS *const lw = w + off;
+    for (int i = threadIdx.x; i < L; i += TX * (L / 8)) twl[i] = ((const C2<TC> *)twL)[i];
+    __syncthreads();
+    using IO = CLoad<TS, TC>;
+    constexpr int SLOTS = 256 / (int)sizeof(C2<TC>);
+    constexpr int SW = (PDO_FFT_STRIDED_SWZ && SLOTS > TX) ? (SLOTS / TX - 1 < 7 ? SLOTS / TX - 1 : 7) : 0;
+    auto bufF = [&](int i) -> C2<TC> & { return tF[(i ^ ((i >> 3) & SW)) * TX + ln]; };
+    auto bufW = [&](int i) -> C2<TC> & { return tW[(i ^ ((i >> 3) & SW)) * TX + ln]; };
+    // forward transforms of the extensions into LDS
+    auto ldF = [&](int i) { return IO::ld(lf, (i < H ? H - 1 - i : i - H) * se); };
+    auto ldW = [&](int i) {
+        if (i < H - 1) {
+            const C2<TC> v = IO::ld(lw, (H - 1 - i) * se);
+            return C2<TC>{-v.x, -v.y};
+        }
+        return IO::ld(lw, (i - (H - 1)) * se);
+    };
+    auto stF = [&](int i, C2<TC> v) { bufF(i) = v; };
+    auto stW = [&](int i, C2<TC> v) { bufW(i) = v; };
+    line_fft<TC, -1, false, L>(ldF, stF, bufF, (const C2<TC> *)twl, jb, act);
+    __syncthreads();
+    line_fft<TC, -1, false, L>(ldW, stW, bufW, (const C2<TC> *)twl, jb, act);
+    __syncthreads();
+    // the pressure and the w update, 8 modes per thread
+    if (act) {
+        const TC fxy = (TC)(ax[a] + ay[blockIdx.y]);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int i = jb * 8 + r;
+            const C2<TC> F = bufF(i), W = bufW(i);
+            const TC ks = fxy + (TC)az[i];
+            const TC kinv = (ks <= TC(1.0e-14)) ? TC(0) : recip(ks);
+            // P = -(F + i cm W) kinv
+            const C2<TC> cw = cmul(C2<TC>{(TC)kcm[2 * i], (TC)kcm[2 * i + 1]}, W);
+            const C2<TC> Pv{-(F.x - cw.y) * kinv, -(F.y + cw.x) * kinv};
+            // W -= i cp P
+            bufF(i) = Pv;
+            if constexpr (UPD) {
+                const C2<TC> cpp = cmul(C2<TC>{(TC)kcp[2 * i], (TC)kcp[2 * i + 1]}, Pv);
+                bufW(i) = C2<TC>{W.x + cpp.y, W.y - cpp.x};
+            }
+        }
+    }
+    __syncthreads();
+    const TC scale = TC(1) / TC(L);
+    auto lbF = [&](int i) { return bufF(i); };
+    auto lbW = [&](int i) { return bufW(i); };
+    auto sgF = [&](int i, C2<TC> v) {
+        if (i >= H) IO::st(lf, (i - H) * se, C2<TC>{v.x * scale, v.y * scale});
+    };
+    auto sgW = [&](int i, C2<TC> v) {
+        if (i >= H - 1) {
+            const int k = i - (H - 1);
+            const bool wall = (k == 0 || k == H);
+            IO::st(lw, k * se, wall ? C2<TC>{TC(0), TC(0)} : C2<TC>{v.x * scale, v.y * scale});
+        }
+    };
+    line_fft<TC, +1, true, L>(lbF, sgF, bufF, (const C2<TC> *)twl, jb, act);
+    if constexpr (UPD) {
+        __syncthreads();
+        line_fft<TC, +1, true, L>(lbW, sgW, bufW, (const C2<TC> *)twl, jb, act);
+    }
+}
+
+template <int L, size_t CE> constexpr int pp_walls_lines()
+{
+    int tx = 512 / (L / 8);
+    if (tx < 1) tx = 1;
+    while (tx > 1 && (2 * (size_t)tx + 1) * L * CE > 64 * 1024) tx >>= 1;
+    return tx;
+}
+
+template <typename TS, typename TC, int L>
+static void launch_pp_walls(TS *f, TS *w, long na, long sb, long se, const TC *tw, const PadeWallTables &t, int nb,
+                            bool upd, hipStream_t st)
+{
+    constexpr size_t ce = 2 * sizeof(TC);
+    // two line tiles and the twiddles: at most 64 KiB
+    constexpr int TX = pp_walls_lines<L, ce>();
+    const size_t lds = (2 * (size_t)TX + 1) * L * ce;
+    dim3 grid((unsigned)((na + TX - 1) / TX), (unsigned)nb);
+    if (upd)
+        hipLaunchKernelGGL((k_pp_walls<TS, TC, L, TX, true>), grid, dim3(TX * (L / 8)), lds, st, f, w, na, sb, (int)se,
+                           tw, t.ax, t.ay, t.az, t.kcm, t.kcp);
+    else
+        hipLaunchKernelGGL((k_pp_walls<TS, TC, L, TX, false>), grid, dim3(TX * (L / 8)), lds, st, f, w, na, sb,
+                           (int)se, tw, t.ax, t.ay, t.az, t.kcm, t.kcp);
+}
+
+template <typename TS, typename TC>
+static hipError_t run_pp_walls_t(const PoissonPlanDev &pl, TS *f, TS *w, const PadeWallTables &t, bool upd,
+                                 hipStream_t st)
+{
+    const int ny = pl.ny, L = pl.nz, nxh = pl.nx / 2 + 1;
+    const Tw<TC> tw(pl);
+    const long na = nxh, sb = nxh, se = (long)nxh * ny;
+    switch (L) {
+    case 8: launch_pp_walls<TS, TC, 8>(f, w, na, sb, se, tw.z, t, ny, upd, st); break;
+    case 16: launch_pp_walls<TS, TC, 16>(f, w, na, sb, se, tw.z, t, ny, upd, st); break;
+    case 32: launch_pp_walls<TS, TC, 32>(f, w, na, sb, se, tw.z, t, ny, upd, st); break;
+    case 64: launch_pp_walls<TS, TC, 64>(f, w, na, sb, se, tw.z, t, ny, upd, st); break;
+    case 128: launch_pp_walls<TS, TC, 128>(f, w, na, sb, se, tw.z, t, ny, upd, st); break;
+    case 256: launch_pp_walls<TS, TC, 256>(f, w, na, sb, se, tw.z, t, ny, upd, st); break;
+    case 512: launch_pp_walls<TS, TC, 512>(f, w, na, sb, se, tw.z, t, ny, upd, st); break;
+    case 1024: launch_pp_walls<TS, TC, 1024>(f, w, na, sb, se, tw.z, t, ny, upd, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t run_pp_walls(const PoissonPlanDev &pl, int mode, void *f, void *w, const PadeWallTables &t, bool upd,
+                        hipStream_t st)
+{
+    switch (mode) {
+    case 0: return run_pp_walls_t<double, double>(pl, (double *)f, (double *)w, t, upd, st);
+    case 1: return run_pp_walls_t<float, float>(pl, (float *)f, (float *)w, t, upd, st);
+    case 2: return run_pp_walls_t<float, double>(pl, (float *)f, (float *)w, t, upd, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 hipError_t run_fft2(const PoissonPlanDev &pl, int mode, const void *in, void *out, int nplanes, hipStream_t st)
 {
     switch (mode) {

@@ -51,6 +51,17 @@ hipError_t run_ifft2(const PoissonPlanDev &pl, int mode, const void *in, void *w
 // z transform, multiply by m, inverse z transform of an unpadded
 // (nx/2+1, ny, nz) spectrum (in may equal out).
 hipError_t run_zmul(const PoissonPlanDev &pl, int mode, const void *in, void *out, const SpecMul &m, hipStream_t st);
+// tables of the PadePoisson solve with walls (device, FP64): ax (nx/2+1),
+// ay (ny), az (2 nz) the squared wavenumbers, kcm and kcp (2 nz complex,
+// interleaved) k3' exp(-+ i k3 dz / 2)
+struct PadeWallTables {
+    const double *ax, *ay, *az, *kcm, *kcp;
+};
+// the extended solve and (upd) w update of PadePoisson with walls, in place
+// on f (nx/2+1, ny, nz) and w (nx/2+1, ny, nz+1); upd false leaves w as it
+// is; the plan's transforms have length 2 nz (its nz)
+hipError_t run_pp_walls(const PoissonPlanDev &pl, int mode, void *f, void *w, const PadeWallTables &t, bool upd,
+                        hipStream_t st);
 
 // Fourier-collocation derivative along dir of real f (n along dir; x:
 // (nx, na, nb), y: (na, ny, nb), z: (na, nb, nz)): the spectrum is multiplied

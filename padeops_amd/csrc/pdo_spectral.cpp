@@ -12,6 +12,7 @@
 #include <cstring>
 
 #include "../../include/padeops_hip.h"
+#include "pdo_kernels.hpp"
 #include "pdo_poisson.hpp"
 #include "pdo_spectral.hpp"
 #include "pdo_system.hpp"
@@ -95,6 +96,9 @@ PadePoissonObject::~PadePoissonObject()
     if (f) (void)hipFree(f);
     if (dp) (void)hipFree(dp);
     if (derz) pdo_op_destroy(derz);
+    if (dz_odd) pdo_op_destroy(dz_odd);
+    if (coef) (void)hipFree(coef);
+    if (cp) (void)hipFree(cp);
 }
 
 int PadePoissonObject::init(SpectralObject *s, double dz)
@@ -127,6 +131,116 @@ int PadePoissonObject::init(SpectralObject *s, double dz)
     std::memset(&zplan, 0, sizeof(zplan));
     zplan.nx = s->nx; zplan.ny = s->ny; zplan.nz = nz; zplan.nxp = s->nxh;
     return fft_tables(zplan, s->nx, s->ny, nz, allocs);
+}
+
+// PadePoisson.F90:181-232 (PeriodicInZ false, no Stokes pressure): the
+// wavenumbers of the 2 nz-point extensions, k3' the cd06stagg modified
+// wavenumber (getmodCD06stagg), the half-cell phases exp(-+ i k3 dz / 2),
+// |k|^2 from GetWaveNums of x and y; derZ of DivergenceCheck (odd ends, :150)
+int PadePoissonObject::init_nonperiodic(SpectralObject *s, double dz)
+{
+    sp = s;
+    nz = s->nz;
+    periodic = false;
+    if (!fft_pow2(2 * nz, 8, 1024)) {
+        set_error("PadePoisson: walls in z need nz a power of two in [4,512]");
+        return PDO_ERR_ARG;
+    }
+    int st = pdo_cd06stagg_init(&dz_odd, nz, dz, 0, 0, 0, 0);
+    if (st) return st;
+    const int L = 2 * nz;
+    std::vector<double> k1 = wavenums(s->nx, s->dx), k2 = wavenums(s->ny, s->dy), k3 = wavenums(L, dz);
+    std::vector<double> ax(s->nxh), ay(s->ny), az(L), cm(2 * L), cp(2 * L);
+    for (int i = 0; i < s->nxh; i++) ax[i] = k1[i] * k1[i];
+    for (int j = 0; j < s->ny; j++) ay[j] = k2[j] * k2[j];
+    for (int k = 0; k < L; k++) {
+        const double km = modified_k(PDO_MODK_CD06STAGG, k3[k], dz), ph = 0.5 * dz * k3[k];
+        az[k] = km * km;
+        cm[2 * k] = km * std::cos(ph);
+        cm[2 * k + 1] = -km * std::sin(ph);
+        cp[2 * k] = km * std::cos(ph);
+        cp[2 * k + 1] = km * std::sin(ph);
+    }
+    const std::vector<double> *src[5] = {&ax, &ay, &az, &cm, &cp};
+    const double **dst[5] = {&wt.ax, &wt.ay, &wt.az, &wt.kcm, &wt.kcp};
+    for (int q = 0; q < 5; q++) {
+        double *p;
+        if ((st = upload_vec(*src[q], &p))) return st;
+        allocs.push_back(p);
+        *dst[q] = p;
+    }
+    std::memset(&zplan2, 0, sizeof(zplan2));
+    zplan2.nx = s->nx; zplan2.ny = s->ny; zplan2.nz = L; zplan2.nxp = s->nxh;
+    return fft_tables(zplan2, s->nx, s->ny, L, allocs);
+}
+
+// PadePoisson.F90:236-307 (computeStokesPressure): per (kx, ky) line
+// lam = |(k1, k2)| of the plain GetWaveNums values and
+// denFact = 1 / (lam sinh(lam Lz) + 1e-13) where lam Lz < 500, else 0, then 0
+// below 1e-16 and at the mean line; the edges linspace(0, Lz, nz + 1)
+// (running sums of the step, gridtools.F90:51-69) and the cells between them
+int PadePoissonObject::set_stokes(double Lz)
+{
+    if (periodic) {
+        set_error("PadePoisson: the Stokes pressure needs walls in z");
+        return PDO_ERR_ARG;
+    }
+    if (!(Lz > 0)) {
+        set_error("PadePoisson: Lz must be positive");
+        return PDO_ERR_ARG;
+    }
+    if (stokes) {
+        set_error("PadePoisson: the Stokes pressure is set already");
+        return PDO_ERR_ARG;
+    }
+    SpectralObject *s = sp;
+    std::vector<double> k1 = wavenums(s->nx, s->dx), k2 = wavenums(s->ny, s->dy);
+    k1.resize(s->nxh);
+    const long np = (long)s->nxh * s->ny;
+    std::vector<double> lam(np), den(np), ze(nz + 1), zc(nz);
+    for (int j = 0; j < s->ny; j++)
+        for (int i = 0; i < s->nxh; i++) {
+            const long q = i + (long)s->nxh * j;
+            const double l = std::sqrt(k1[i] * k1[i] + k2[j] * k2[j]);
+            double d = (l * Lz < 500.0) ? 1.0 / (l * std::sinh(l * Lz) + 1.0e-13) : 0.0;
+            if (d < 1.0e-16) d = 0.0;
+            lam[q] = l;
+            den[q] = d;
+        }
+    den[0] = 0.0;
+    const double step = (Lz - 0.0) / ((double)(nz + 1) - 1.0);
+    ze[0] = 0.0;
+    for (int k = 1; k <= nz; k++) ze[k] = ze[k - 1] + step;
+    for (int k = 0; k < nz; k++) zc[k] = 0.5 * (ze[k] + ze[k + 1]);
+    const std::vector<double> *src[6] = {&k1, &k2, &lam, &den, &zc, &ze};
+    const double **dst[6] = {&stt.k1, &stt.k2, &stt.lam, &stt.den, &stt.zc, &stt.ze};
+    for (int q = 0; q < 6; q++) {
+        double *p;
+        int st = upload_vec(*src[q], &p);
+        if (st) return st;
+        allocs.push_back(p);
+        *dst[q] = p;
+    }
+    hipError_t e = hipMalloc((void **)&coef, (size_t)np * 4 * sizeof(double));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(Stokes coefficients)");
+    stt.Lz = Lz;
+    stt.nxh = s->nxh;
+    stt.ny = s->ny;
+    stt.nz = nz;
+    stokes = true;
+    return 0;
+}
+
+int PadePoissonObject::ensure_copies(size_t bytes)
+{
+    if (cp_bytes >= bytes) return 0;
+    if (cp) (void)hipFree(cp);
+    cp = nullptr;
+    cp_bytes = 0;
+    hipError_t e = hipMalloc(&cp, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(getPressure copies)");
+    cp_bytes = bytes;
+    return 0;
 }
 
 int PadePoissonObject::ensure(size_t fb, size_t db)
@@ -271,28 +385,128 @@ int pdo_padepoisson_init_periodic(pdo_padepoisson *h, pdo_spectral sp, double dz
     return 0;
 }
 
-int pdo_padepoisson_projection(pdo_padepoisson h, int prec, void *uhat, void *vhat, void *what, void *stream)
+int pdo_padepoisson_init_nonperiodic(pdo_padepoisson *h, pdo_spectral sp, double dz)
 {
-    if (!h || !uhat || !vhat || !what) { set_error("invalid argument"); return PDO_ERR_ARG; }
-    const int mode = fft_mode(prec);
-    if (mode < 0) { set_error("unknown precision mode"); return PDO_ERR_ARG; }
+    if (!h || !sp) { set_error("invalid argument"); return PDO_ERR_ARG; }
+    *h = nullptr;
+    if (sp->periodic_z) {
+        set_error("PadePoisson with walls: the spectral object must not be periodic in z");
+        return PDO_ERR_ARG;
+    }
+    auto *o = new pdo_padepoisson_s();
+    int st = o->init_nonperiodic(sp, dz);
+    if (st) { delete o; return st; }
+    *h = o;
+    return 0;
+}
+
+// the walls solve of PressureProjection (PadePoisson.F90:434-624) and
+// getPressure (:752-898): Step 0 the Stokes step in place on (u, v, w), from
+// where on k1, k2 are the plain GetWaveNums values (k1inZ, k2inZ); Step 1
+// f = i (k1 u + k2 v); Steps 3-8 the extensions, transforms, pressure,
+// (upd) w update and walls; Step 9 (upd) u -= i k1 p, v -= i k2 p.  The
+// pressure stays in h->f (with the Stokes part when pressure is set).
+static int walls_solve(pdo_padepoisson h, int mode, void *uhat, void *vhat, void *what, bool upd, bool pressure,
+                       hipStream_t str)
+{
+    SpectralObject *s = h->sp;
+    const size_t W = mode == 0 ? 8 : 4;
+    const long ncell = (long)s->nxh * s->ny * h->nz;
+    int st = h->ensure((size_t)ncell * 2 * W, 0);
+    if (st) return st;
+    const double *k1 = h->stokes ? h->stt.k1 : s->d_k1, *k2 = h->stokes ? h->stt.k2 : s->d_k2;
+    hipError_t e;
+    if (h->stokes && (e = launch_stokes(mode, uhat, vhat, what, h->coef, h->stt, str)) != hipSuccess)
+        return hip_fail(e, "Stokes pressure");
+    if ((e = launch_div_spec(mode, uhat, vhat, nullptr, h->f, k1, k2, s->nxh, s->ny, ncell, str)) != hipSuccess)
+        return hip_fail(e, "projection divergence");
+    if ((e = run_pp_walls(h->zplan2, mode, h->f, what, h->wt, upd, str)) != hipSuccess)
+        return hip_fail(e, "projection solve");
+    if (upd && (e = launch_proj_update(mode, uhat, vhat, what, h->f, nullptr, k1, k2, s->nxh, s->ny, ncell, ncell,
+                                       str)) != hipSuccess)
+        return hip_fail(e, "projection update");
+    if (pressure && h->stokes && (e = launch_stokes_pressure(mode, h->f, h->coef, h->stt, str)) != hipSuccess)
+        return hip_fail(e, "Stokes pressure");
+    return 0;
+}
+
+static int projection_walls(pdo_padepoisson h, int mode, void *uhat, void *vhat, void *what, hipStream_t str)
+{
+    return walls_solve(h, mode, uhat, vhat, what, true, false, str);
+}
+
+// PeriodicProjection (PadePoisson.F90:386-432) and Periodic_getPressure
+// (:716-750): f = ddz_E2C(w) + i (k1 u + k2 v), the z solve
+// p = z-inverse(-1/|k|^2 z-forward(f)) / nz into h->f; upd: w -= ddz_C2E(p),
+// u -= i k1 p, v -= i k2 p
+static int periodic_solve(pdo_padepoisson h, int mode, int prec, void *uhat, void *vhat, void *what, bool upd,
+                          hipStream_t str)
+{
     SpectralObject *s = h->sp;
     const size_t W = mode == 0 ? 8 : 4;
     const long ncell = (long)s->nxh * s->ny * h->nz, nedge = (long)s->nxh * s->ny * (h->nz + 1);
     int st = h->ensure((size_t)ncell * 2 * W, (size_t)nedge * 2 * W);
     if (st) return st;
-    hipStream_t str = (hipStream_t)stream;
-    // f = d(w)/dz (E2C) + i (k1 u + k2 v)
     if ((st = apply(h->derz, KIND_STG_D1_E2C, 0, 0, 3, prec, what, h->f, 2L * s->nxh, s->ny, str))) return st;
     hipError_t e = launch_div_spec(mode, uhat, vhat, h->f, h->f, s->d_k1, s->d_k2, s->nxh, s->ny, ncell, str);
     if (e != hipSuccess) return hip_fail(e, "projection divergence");
-    // p = z-inverse( -1/|k|^2 z-forward(f) ) / nz
     const SpecMul m{h->d_ax, h->d_ay, h->d_az, 1, 1.0 / h->nz};
     if ((e = run_zmul(h->zplan, mode, h->f, h->f, m, str)) != hipSuccess) return hip_fail(e, "projection solve");
-    // w -= dp/dz (C2E); u -= i k1 p; v -= i k2 p
+    if (!upd) return 0;
     if ((st = apply(h->derz, KIND_STG_D1_C2E, 0, 0, 3, prec, h->f, h->dp, 2L * s->nxh, s->ny, str))) return st;
     e = launch_proj_update(mode, uhat, vhat, what, h->f, h->dp, s->d_k1, s->d_k2, s->nxh, s->ny, ncell, nedge, str);
     return hip_status(e, "projection update");
+}
+
+int pdo_padepoisson_projection(pdo_padepoisson h, int prec, void *uhat, void *vhat, void *what, void *stream)
+{
+    if (!h || !uhat || !vhat || !what) { set_error("invalid argument"); return PDO_ERR_ARG; }
+    const int mode = fft_mode(prec);
+    if (mode < 0) { set_error("unknown precision mode"); return PDO_ERR_ARG; }
+    if (!h->periodic) return projection_walls(h, mode, uhat, vhat, what, (hipStream_t)stream);
+    return periodic_solve(h, mode, prec, uhat, vhat, what, true, (hipStream_t)stream);
+}
+
+int pdo_padepoisson_set_stokes(pdo_padepoisson h, double Lz)
+{
+    if (!h) { set_error("invalid argument"); return PDO_ERR_ARG; }
+    return h->set_stokes(Lz);
+}
+
+int pdo_padepoisson_pressure(pdo_padepoisson h, int prec, void *uhat, void *vhat, void *what, void *pressure,
+                             int update, void *stream)
+{
+    if (!h || !uhat || !vhat || !what || !pressure || (update != 0 && update != 1)) {
+        set_error("invalid argument");
+        return PDO_ERR_ARG;
+    }
+    const int mode = fft_mode(prec);
+    if (mode < 0) { set_error("unknown precision mode"); return PDO_ERR_ARG; }
+    hipStream_t str = (hipStream_t)stream;
+    int st;
+    if (h->periodic) {
+        st = periodic_solve(h, mode, prec, uhat, vhat, what, update == 1, str);
+    } else if (update || !h->stokes) {
+        st = walls_solve(h, mode, uhat, vhat, what, update == 1, true, str);
+    } else {
+        // getPressure with the Stokes pressure: its step works on copies
+        SpectralObject *s = h->sp;
+        const size_t W = mode == 0 ? 8 : 4;
+        const size_t bc = (size_t)s->nxh * s->ny * h->nz * 2 * W, be = (size_t)s->nxh * s->ny * (h->nz + 1) * 2 * W;
+        if ((st = h->ensure_copies(2 * bc + be))) return st;
+        char *c = (char *)h->cp;
+        const void *srcs[3] = {uhat, vhat, what};
+        const size_t sizes[3] = {bc, bc, be};
+        size_t off = 0;
+        for (int q = 0; q < 3; q++) {
+            hipError_t e = hipMemcpyAsync(c + off, srcs[q], sizes[q], hipMemcpyDeviceToDevice, str);
+            if (e != hipSuccess) return hip_fail(e, "getPressure copies");
+            off += sizes[q];
+        }
+        st = walls_solve(h, mode, c, c + bc, c + 2 * bc, false, true, str);
+    }
+    if (st) return st;
+    return pdo_spectral_ifft(static_cast<pdo_spectral>(h->sp), prec, h->f, pressure, 0, stream);
 }
 
 int pdo_padepoisson_divergence(pdo_padepoisson h, int prec, const void *uhat, const void *vhat, const void *what,
@@ -307,7 +521,11 @@ int pdo_padepoisson_divergence(pdo_padepoisson h, int prec, const void *uhat, co
     int st = h->ensure((size_t)ncell * 2 * W, (size_t)nedge * 2 * W);
     if (st) return st;
     hipStream_t str = (hipStream_t)stream;
-    if ((st = apply(h->derz, KIND_STG_D1_E2C, 0, 0, 3, prec, what, h->f, 2L * s->nxh, s->ny, str))) return st;
+    // the z term: periodic cd06stagg, or with walls the odd-ended one of the
+    // reference's derZ (PadePoisson.F90:150-151)
+    if ((st = apply(h->periodic ? h->derz : h->dz_odd, KIND_STG_D1_E2C, 0, 0, 3, prec, what, h->f, 2L * s->nxh,
+                    s->ny, str)))
+        return st;
     hipError_t e = launch_div_spec(mode, uhat, vhat, h->f, h->f, s->d_k1, s->d_k2, s->nxh, s->ny, ncell, str);
     if (e != hipSuccess) return hip_fail(e, "divergence");
     return pdo_spectral_ifft(static_cast<pdo_spectral>(s), prec, h->f, divergence, 0, stream);

@@ -1,7 +1,8 @@
 // Pointwise kernels on half spectra (nxh, ny, nplanes) of interleaved complex
 // values: multiplication by i k (spectral.F90:235-312), the 2/3-rule mask
-// (spectral.F90:314-341) and the two pointwise steps of the periodic pressure
-// projection and divergence check (PadePoisson.F90:386-432, 1165-1244).
+// (spectral.F90:314-341), the two pointwise steps of the pressure projection
+// and divergence check (PadePoisson.F90:386-432, 1165-1244) and the Stokes
+// pressure of the projection with walls (:236-384, 641-714).
 // One thread per complex value, grid-stride; every kernel is HBM-bound.
 #include <hip/hip_runtime.h>
 
@@ -54,7 +55,7 @@ __global__ void __launch_bounds__(256) k_mask2(TS *f, const double *__restrict__
     }
 }
 
-// f = d + i (k1 u + k2 v)
+// f = d + i (k1 u + k2 v) (d null: zero)
 template <typename TS, typename TC>
 __global__ void __launch_bounds__(256) k_div_spec(const TS *u, const TS *v, const TS *d, TS *f,
                                                  const double *__restrict__ k1, const double *__restrict__ k2, int nxh,
@@ -64,20 +65,23 @@ __global__ void __launch_bounds__(256) k_div_spec(const TS *u, const TS *v, cons
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
         const long i = t % nxh, j = (t / nxh) % ny;
         const TC a = (TC)k1[i], b = (TC)k2[j];
-        const V uu = ((const V *)u)[t], vv = ((const V *)v)[t], dd = ((const V *)d)[t];
+        const V uu = ((const V *)u)[t], vv = ((const V *)v)[t];
+        const V dd = d ? ((const V *)d)[t] : V{(TS)0, (TS)0};
         const TC sr = a * (TC)uu.x + b * (TC)vv.x, si = a * (TC)uu.y + b * (TC)vv.y;
         ((V *)f)[t] = V{(TS)((TC)dd.x - si), (TS)((TC)dd.y + sr)};
     }
 }
 
-// u -= i k1 p, v -= i k2 p on the nz cell planes; w -= dp on the nz+1 edge planes
+// u -= i k1 p, v -= i k2 p on the nz cell planes; w -= dp on the nz+1 edge
+// planes (dp null: w untouched)
 template <typename TS, typename TC>
 __global__ void __launch_bounds__(256) k_proj_update(TS *u, TS *v, TS *w, const TS *p, const TS *dp,
                                                     const double *__restrict__ k1, const double *__restrict__ k2,
                                                     int nxh, int ny, long ncell, long nedge)
 {
     using V = typename CV<TS>::V;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nedge; t += (long)gridDim.x * blockDim.x) {
+    const long n = dp ? nedge : ncell;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
         if (t < ncell) {
             const long i = t % nxh, j = (t / nxh) % ny;
             const TC a = (TC)k1[i], b = (TC)k2[j];
@@ -90,6 +94,7 @@ __global__ void __launch_bounds__(256) k_proj_update(TS *u, TS *v, TS *w, const 
             ((V *)u)[t] = uu;
             ((V *)v)[t] = vv;
         }
+        if (!dp) continue;
         const V dd = ((const V *)dp)[t];
         V ww = ((V *)w)[t];
         ww.x = (TS)((TC)ww.x - (TC)dd.x);
@@ -166,6 +171,125 @@ hipError_t launch_proj_update(int mode, void *u, void *v, void *w, const void *p
     case 2: return proj_update_t<float, double>(u, v, w, p, dp, k1, k2, nxh, ny, ncell, nedge, st);
     }
     return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// Stokes pressure with walls (PadePoisson.F90:236-307 tables, 320-384 and
+// 641-714 the step), one thread per (kx, ky) line.  With lam = |(k1, k2)|
+// (plain GetWaveNums values) and denFact d of the line:
+//   chat_b = -w(bottom) d; chat_t = (w(top) - chat_b sinh_bot(Lz)) d
+//   u -= k1 (i chat_b cosh_bot + i chat_t cosh_top) on the cells (v with k2)
+//   w -= chat_b sinh_bot + chat_t sinh_top on the inner edges, 0 on the walls
+// cosh_bot(z) = cosh(lam (Lz - z)), cosh_top(z) = cosh(lam z) (caps 4e13,
+// 1e13 from an argument of 32 on), sinh_bot(z) = -lam sinh(lam (Lz - z)),
+// sinh_top(z) = lam sinh(lam z) (caps -+4e13): the reference's tables,
+// evaluated on the fly.  Arithmetic in FP64 for every storage type.  coef
+// (4 doubles per line) keeps chat_b, chat_t for the pressure.
+// ---------------------------------------------------------------------------
+__device__ inline double st_cosh_bot(double l, double Lz, double z)
+{
+    const double t = l * (Lz - z);
+    return t < 32.0 ? cosh(t) : 4.0e13;
+}
+__device__ inline double st_cosh_top(double l, double z)
+{
+    const double t = l * z;
+    return t < 32.0 ? cosh(t) : 1.0e13;
+}
+__device__ inline double st_sinh_bot(double l, double Lz, double z)
+{
+    const double t = l * (Lz - z);
+    return t < 32.0 ? -l * sinh(t) : -4.0e13;
+}
+__device__ inline double st_sinh_top(double l, double z)
+{
+    const double t = l * z;
+    return t < 32.0 ? l * sinh(t) : 4.0e13;
+}
+
+template <typename TS>
+__global__ void __launch_bounds__(256) k_stokes(TS *u, TS *v, TS *w, double *coef, StokesTables t)
+{
+    using V = typename CV<TS>::V;
+    const long np = (long)t.nxh * t.ny;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const double a = t.k1[q % t.nxh], b = t.k2[q / t.nxh], l = t.lam[q], d = t.den[q];
+    V *wq = (V *)w + q;
+    const V w0 = wq[0], wn = wq[(long)t.nz * np];
+    const double cbr = -(double)w0.x * d, cbi = -(double)w0.y * d;
+    const double sbn = st_sinh_bot(l, t.Lz, t.ze[t.nz]);
+    const double ctr = ((double)wn.x - cbr * sbn) * d, cti = ((double)wn.y - cbi * sbn) * d;
+    coef[4 * q] = cbr;
+    coef[4 * q + 1] = cbi;
+    coef[4 * q + 2] = ctr;
+    coef[4 * q + 3] = cti;
+    V *uq = (V *)u + q, *vq = (V *)v + q;
+    for (int k = 0; k < t.nz; ++k) {
+        const double z = t.zc[k], hb = st_cosh_bot(l, t.Lz, z), ht = st_cosh_top(l, z);
+        // i chat cosh
+        const double pbr = -cbi * hb, pbi = cbr * hb, ptr = -cti * ht, pti = ctr * ht;
+        V uu = uq[k * np], vv = vq[k * np];
+        uu.x = (TS)(((double)uu.x - a * pbr) - a * ptr);
+        uu.y = (TS)(((double)uu.y - a * pbi) - a * pti);
+        vv.x = (TS)(((double)vv.x - b * pbr) - b * ptr);
+        vv.y = (TS)(((double)vv.y - b * pbi) - b * pti);
+        uq[k * np] = uu;
+        vq[k * np] = vv;
+    }
+    {
+        const double s0 = st_sinh_top(l, t.ze[0]);
+        wq[0] = V{(TS)(0.0 - ctr * s0), (TS)(0.0 - cti * s0)};
+    }
+    for (int k = 1; k < t.nz; ++k) {
+        const double z = t.ze[k], sb = st_sinh_bot(l, t.Lz, z), stp = st_sinh_top(l, z);
+        V ww = wq[k * np];
+        ww.x = (TS)(((double)ww.x - cbr * sb) - ctr * stp);
+        ww.y = (TS)(((double)ww.y - cbi * sb) - cti * stp);
+        wq[k * np] = ww;
+    }
+    wq[(long)t.nz * np] = V{(TS)0, (TS)0};
+}
+
+// p += i chat_b cosh_bot + i chat_t cosh_top on the nz cell planes (the
+// reference's pressure adds phat_z1 + phat_z2, :882-890)
+template <typename TS>
+__global__ void __launch_bounds__(256) k_stokes_pressure(TS *p, const double *coef, StokesTables t)
+{
+    using V = typename CV<TS>::V;
+    const long np = (long)t.nxh * t.ny, n = np * t.nz;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const long q = e % np;
+        const int k = (int)(e / np);
+        const double l = t.lam[q], z = t.zc[k], hb = st_cosh_bot(l, t.Lz, z), ht = st_cosh_top(l, z);
+        const double *c = coef + 4 * q;
+        V pp = ((V *)p)[e];
+        pp.x = (TS)(((double)pp.x + (-c[1] * hb)) + (-c[3] * ht));
+        pp.y = (TS)(((double)pp.y + c[0] * hb) + c[2] * ht);
+        ((V *)p)[e] = pp;
+    }
+}
+
+hipError_t launch_stokes(int mode, void *u, void *v, void *w, double *coef, const StokesTables &t, hipStream_t st)
+{
+    const long np = (long)t.nxh * t.ny;
+    const dim3 grid((unsigned)((np + 255) / 256));
+    if (mode == 0)
+        hipLaunchKernelGGL((k_stokes<double>), grid, dim3(256), 0, st, (double *)u, (double *)v, (double *)w, coef,
+                           t);
+    else
+        hipLaunchKernelGGL((k_stokes<float>), grid, dim3(256), 0, st, (float *)u, (float *)v, (float *)w, coef, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_stokes_pressure(int mode, void *p, const double *coef, const StokesTables &t, hipStream_t st)
+{
+    const long n = (long)t.nxh * t.ny * t.nz;
+    if (mode == 0)
+        hipLaunchKernelGGL((k_stokes_pressure<double>), dim3(grid_for(n)), dim3(256), 0, st, (double *)p, coef, t);
+    else
+        hipLaunchKernelGGL((k_stokes_pressure<float>), dim3(grid_for(n)), dim3(256), 0, st, (float *)p, coef, t);
+    return hipGetLastError();
 }
 
 }  // namespace pdo

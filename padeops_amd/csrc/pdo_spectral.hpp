@@ -20,6 +20,19 @@ hipError_t launch_div_spec(int mode, const void *u, const void *v, const void *d
 hipError_t launch_proj_update(int mode, void *u, void *v, void *w, const void *p, const void *dp, const double *k1,
                               const double *k2, int nxh, int ny, long ncell, long nedge, hipStream_t st);
 
+// Stokes pressure of PadePoisson with walls (device tables, FP64): k1
+// (nx/2+1) and k2 (ny) the plain GetWaveNums values, lam = |(k1, k2)| and
+// denFact per (kx, ky) line, the cell and edge z (nz, nz+1), Lz
+struct StokesTables {
+    const double *k1, *k2, *lam, *den, *zc, *ze;
+    double Lz;
+    int nxh, ny, nz;
+};
+// the Stokes step in place on u, v (nz planes) and w (nz+1 planes), chat of
+// both walls into coef (4 doubles per line); the Stokes pressure added to p
+hipError_t launch_stokes(int mode, void *u, void *v, void *w, double *coef, const StokesTables &t, hipStream_t st);
+hipError_t launch_stokes_pressure(int mode, void *p, const double *coef, const StokesTables &t, hipStream_t st);
+
 struct SpectralObject {
     int nx = 0, ny = 0, nz = 0, nxh = 0;
     double dx = 1, dy = 1, dz = 1;
@@ -48,7 +61,23 @@ struct PadePoissonObject {
     std::vector<void *> allocs;
     void *f = nullptr, *dp = nullptr;     // (nxh, ny, nz) and (nxh, ny, nz+1) complex work
     size_t f_bytes = 0, dp_bytes = 0;
+    // walls in z (init_nonperiodic): transforms of the 2 nz-point
+    // extensions and their tables; the odd-ended cd06stagg of DivergenceCheck
+    bool periodic = true;
+    pdo_op_s *dz_odd = nullptr;
+    PoissonPlanDev zplan2{};
+    PadeWallTables wt{};
+    // the Stokes pressure (walls only): its tables, chat of the last call per
+    // line, and copies of u, v, w for getPressure (which leaves its inputs)
+    bool stokes = false;
+    StokesTables stt{};
+    double *coef = nullptr;
+    void *cp = nullptr;
+    size_t cp_bytes = 0;
     int init(SpectralObject *sp, double dz);
+    int init_nonperiodic(SpectralObject *sp, double dz);
+    int set_stokes(double Lz);
+    int ensure_copies(size_t bytes);
     int ensure(size_t fb, size_t db);
     ~PadePoissonObject();
 };

@@ -162,6 +162,12 @@ module padeops_hip_mod
             type(c_ptr), value :: sp
             real(c_double), value :: dz
         end function
+        integer(c_int) function c_pp_init_np(h, sp, dz) bind(C, name="pdo_padepoisson_init_nonperiodic")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr) :: h
+            type(c_ptr), value :: sp
+            real(c_double), value :: dz
+        end function
         integer(c_int) function c_pp_project(h, prec, uh, vh, wh, stream) bind(C, name="pdo_padepoisson_projection")
             import :: c_ptr, c_int
             type(c_ptr), value :: h, uh, vh, wh, stream
@@ -171,6 +177,16 @@ module padeops_hip_mod
             import :: c_ptr, c_int
             type(c_ptr), value :: h, uh, vh, wh, div, stream
             integer(c_int), value :: prec
+        end function
+        integer(c_int) function c_pp_set_stokes(h, Lz) bind(C, name="pdo_padepoisson_set_stokes")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h
+            real(c_double), value :: Lz
+        end function
+        integer(c_int) function c_pp_pressure(h, prec, uh, vh, wh, p, upd, stream) bind(C, name="pdo_padepoisson_pressure")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: h, uh, vh, wh, p, stream
+            integer(c_int), value :: prec, upd
         end function
         integer(c_int) function c_pp_destroy(h) bind(C, name="pdo_padepoisson_destroy")
             import :: c_ptr, c_int
@@ -400,7 +416,7 @@ module padeops_hip_mod
         generic :: interpz_E2C => interpz_E2C_real, interpz_E2C_cmplx
     end type
 
-    ! padepoisson, periodic in z (PadePoisson.F90:20-72)
+    ! padepoisson (PadePoisson.F90:20-72), periodic in z or with walls
     type, public :: padepoisson
         type(c_ptr) :: h = c_null_ptr
         integer :: nx_in = 0, ny_in = 0, nz_in = 0, nxE_in = 0, nyE_in = 0, nzE_in = 0, nx = 0, ny = 0, nz = 0
@@ -410,6 +426,8 @@ module padeops_hip_mod
     contains
         procedure :: init => pp_init
         procedure :: PressureProjection => pp_projection
+        procedure :: getPressure => pp_get_pressure
+        procedure :: getPressureAndUpdateRHS => pp_get_pressure_update
         procedure :: DivergenceCheck => pp_divergence
         procedure :: destroy => pp_destroy
     end type
@@ -1701,7 +1719,8 @@ contains
                               this%sp_gp%zsz(1), this%sp_gp%zsz(2), 1, this%stream))
     end subroutine
 
-    ! ---- padepoisson, periodic in z (PadePoisson.F90:130-180, 386-432, 1165-1244)
+    ! ---- padepoisson (PadePoisson.F90:130-1244): periodic in z, or with walls
+    ! and optionally the Stokes pressure
     subroutine pp_init(this, dx, dy, dz, sp, spE, computeStokesPressure, Lz, storePressure, gpC, derivZ, PeriodicInZ)
         class(padepoisson), intent(inout) :: this
         real(rkind), intent(in) :: dx, dy, dz
@@ -1711,16 +1730,22 @@ contains
         type(decomp_info), intent(in), target :: gpC
         type(Pade6stagg), intent(in), target :: derivZ
         logical, intent(in) :: PeriodicInZ
-        if (.not. PeriodicInZ) error stop "padepoisson: only the periodic-in-z projection is supported"
-        if (computeStokesPressure) error stop "padepoisson: the Stokes pressure is not supported"
         if (derivZ%scheme /= cd06_scheme) error stop "padepoisson: derivZ must use the cd06 scheme"
         if (present(Lz) .or. storePressure .or. dx < 0 .or. dy < 0 .or. gpC%xsz(1) < 0) continue
         call pp_destroy(this)
-        this%PeriodicInZ = .true.
+        this%PeriodicInZ = PeriodicInZ
         this%nx = sp%nx_g; this%ny = sp%ny_g; this%nz = sp%nz_g
         this%nx_in = sp%nx_c; this%ny_in = sp%ny_c; this%nz_in = sp%nz_c
         this%nxE_in = spE%nx_c; this%nyE_in = spE%ny_c; this%nzE_in = spE%nz_c
-        call check(c_pp_init(this%h, sp%h, dz))
+        if (PeriodicInZ) then
+            call check(c_pp_init(this%h, sp%h, dz))
+        else
+            call check(c_pp_init_np(this%h, sp%h, dz))
+            if (computeStokesPressure) then
+                if (.not. present(Lz)) error stop "padepoisson: the Stokes pressure needs Lz"
+                call check(c_pp_set_stokes(this%h, Lz))
+            end if
+        end if
     end subroutine
 
     subroutine pp_projection(this, uhat, vhat, what)
@@ -1728,6 +1753,27 @@ contains
         complex(rkind), dimension(this%nx_in, this%ny_in, this%nz_in), intent(inout), target :: uhat, vhat
         complex(rkind), dimension(this%nxE_in, this%nyE_in, this%nzE_in), intent(inout), target :: what
         call check(c_pp_project(this%h, this%prec, c_loc(uhat), c_loc(vhat), c_loc(what), this%stream))
+    end subroutine
+
+    ! getPressure (PadePoisson.F90:752-898): the velocities stay as they are
+    subroutine pp_get_pressure(this, uhat, vhat, what, pressure)
+        class(padepoisson), intent(inout) :: this
+        complex(rkind), dimension(this%nx_in, this%ny_in, this%nz_in), intent(in), target :: uhat, vhat
+        complex(rkind), dimension(this%nxE_in, this%nyE_in, this%nzE_in), intent(in), target :: what
+        real(rkind), dimension(this%nx, this%ny, this%nz), intent(inout), target :: pressure
+        call check(c_pp_pressure(this%h, this%prec, c_loc(uhat), c_loc(vhat), c_loc(what), c_loc(pressure), 0, &
+                                 this%stream))
+    end subroutine
+
+    ! getPressureAndUpdateRHS (PadePoisson.F90:952-1163): the projection and
+    ! its pressure
+    subroutine pp_get_pressure_update(this, uhat, vhat, what, pressure)
+        class(padepoisson), intent(inout) :: this
+        complex(rkind), dimension(this%nx_in, this%ny_in, this%nz_in), intent(inout), target :: uhat, vhat
+        complex(rkind), dimension(this%nxE_in, this%nyE_in, this%nzE_in), intent(inout), target :: what
+        real(rkind), dimension(this%nx, this%ny, this%nz), intent(inout), target :: pressure
+        call check(c_pp_pressure(this%h, this%prec, c_loc(uhat), c_loc(vhat), c_loc(what), c_loc(pressure), 1, &
+                                 this%stream))
     end subroutine
 
     ! steps 1-3 of DivergenceCheck: the divergence field (the reference then

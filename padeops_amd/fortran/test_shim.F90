@@ -18,7 +18,10 @@
 !   padepoisson                     Pade6stagg ddz_C2E (real and complex) with
 !                                   the staggered wavenumber; PressureProjection
 !                                   of sin(x) + sin(y) leaves sin(y), with zero
-!                                   DivergenceCheck
+!                                   DivergenceCheck; the same with walls in z
+!                                   (w zero on both walls); getPressure and
+!                                   getPressureAndUpdateRHS (walls, Stokes) of
+!                                   (sin x, 0, 0) give -cos x and u = 0
 !   padeops_hip_decomp              decomp_info sizes, transpose_x_to_y copy
 ! Prints PASS or stops with a nonzero code.
 module shim_callbacks
@@ -62,7 +65,7 @@ program test_shim
     type(poissonperiodic) :: poiss
     type(derivatives) :: der
     type(filters) :: fil
-    type(spectral) :: spC, spE
+    type(spectral) :: spC, spE, spW
     type(Pade6stagg) :: p6
     type(padepoisson) :: pp
     type(decomp_info) :: gpC, gpE
@@ -370,6 +373,89 @@ program test_shim
     print '(a,es10.3)', "padepoisson DivergenceCheck after projection: ", err
     if (err > 1d-12) error stop 55
     call pp%destroy()
+
+    ! walls in z (PeriodicInZ false): sin(y) survives again, w =
+    ! cos(2y) sin(pi z / Lz) (zero on the walls, as the extension assumes)
+    ! only adds divergence in the (0, 2) modes; w stays zero on both walls and
+    ! DivergenceCheck (odd-ended ddz_E2C) vanishes
+    call spW%init("x", m, m, m, dx, dx, dz, "four", "none", 2, fixOddball=.false., init_periodicInZ=.false.)
+    do j = 1, m
+        do i = 1, m
+            g(i, j, :) = sin((i - 1) * dx) + sin((j - 1) * dx)
+        end do
+    end do
+    do k = 1, m + 1
+        do j = 1, m
+            gE(:, j, k) = cos(2 * (j - 1) * dx) * sin(pi * (k - 1) / m)
+        end do
+    end do
+    gr = 0
+    call pdo_to_device(pa, c_loc(g), rb)
+    call spW%fft(ad, cu)
+    call pdo_to_device(pa, c_loc(gr), rb)
+    call spW%fft(ad, cv)
+    call pdo_to_device(pe, c_loc(gE), rbE)
+    call spE%fft(ed, cw)
+    call pp%init(dx, dx, dz, spW, spE, .false., 2 * pi, .false., gpC, p6, .false.)
+    call pp%PressureProjection(cu, cv, cw)
+    call spW%ifft(cu, bd)
+    call pdo_synchronize(c_null_ptr)
+    call pdo_to_host(c_loc(gr), pb, rb)
+    call pdo_to_host(c_loc(zE), pcw, cbE)
+    err = 0
+    do j = 1, m
+        err = max(err, maxval(abs(gr(:, j, :) - sin((j - 1) * dx))))
+    end do
+    print '(a,es10.3)', "padepoisson with walls: PressureProjection of sin(x) + sin(y): ", err
+    if (err > 1d-12) error stop 56
+    err = max(maxval(abs(zE(:, :, 1))), maxval(abs(zE(:, :, m + 1))))
+    print '(a,es10.3)', "padepoisson with walls: w on the walls: ", err
+    if (err /= 0) error stop 57
+    call pp%DivergenceCheck(cu, cv, cw, bd)
+    call pdo_synchronize(c_null_ptr)
+    call pdo_to_host(c_loc(gr), pb, rb)
+    err = maxval(abs(gr))
+    print '(a,es10.3)', "padepoisson with walls: DivergenceCheck after projection: ", err
+    if (err > 1d-11) error stop 58
+    call pp%destroy()
+
+    ! getPressure / getPressureAndUpdateRHS with walls and the Stokes
+    ! pressure: (sin x, 0, 0) is the gradient of -cos x, so both pressures are
+    ! -cos x and the update leaves u = 0 (no wall velocity: no Stokes part)
+    call pp%init(dx, dx, dz, spW, spE, .true., m * dz, .false., gpC, p6, .false.)
+    do i = 1, m
+        g(i, :, :) = sin((i - 1) * dx)
+    end do
+    gr = 0
+    gE = 0
+    call pdo_to_device(pa, c_loc(g), rb)
+    call spW%fft(ad, cu)
+    call pdo_to_device(pa, c_loc(gr), rb)
+    call spW%fft(ad, cv)
+    call pdo_to_device(pe, c_loc(gE), rbE)
+    call spE%fft(ed, cw)
+    call pp%getPressure(cu, cv, cw, bd)
+    call pdo_synchronize(c_null_ptr)
+    call pdo_to_host(c_loc(gr), pb, rb)
+    err = 0
+    do i = 1, m
+        err = max(err, maxval(abs(gr(i, :, :) + cos((i - 1) * dx))))
+    end do
+    print '(a,es10.3)', "padepoisson with walls + Stokes: getPressure of (sin x, 0, 0) vs -cos x: ", err
+    if (err > 1d-12) error stop 59
+    call pp%getPressureAndUpdateRHS(cu, cv, cw, bd)
+    call pdo_synchronize(c_null_ptr)
+    call pdo_to_host(c_loc(gr), pb, rb)
+    err = 0
+    do i = 1, m
+        err = max(err, maxval(abs(gr(i, :, :) + cos((i - 1) * dx))))
+    end do
+    call pdo_to_host(c_loc(zr), pcu, cb)
+    err = max(err, maxval(abs(zr(:, :, 1:m))))
+    print '(a,es10.3)', "padepoisson with walls + Stokes: getPressureAndUpdateRHS (pressure, u): ", err
+    if (err > 1d-12) error stop 60
+    call pp%destroy()
+    call spW%destroy()
     call p6%destroy()
     call spC%destroy()
     call spE%destroy()

@@ -1,0 +1,208 @@
+"""GPU parity of the PadePoisson projection with walls in z
+(pdo_padepoisson_init_nonperiodic, pdo_padepoisson_set_stokes) and of
+getPressure / getPressureAndUpdateRHS (pdo_padepoisson_pressure, periodic and
+walls) against the oracle's restatement of the reference
+(oracle/spectral.py PadePoissonWalls, PadePoissonPeriodic): FP64 to 1e-12
+relative; FP32 + refinement against the FP64 GPU result to 2e-6; the
+DivergenceCheck field of the projected velocity vanishes and w is zero on
+both walls; getPressure leaves its inputs bit for bit as they were.  The
+Stokes cases include a box (Lz = 2, 64 x 32 x 16) whose lines reach the
+reference's cosh / sinh caps."""
+import numpy as np
+import pytest
+
+import padeops_amd as P
+from fields import rel_l2
+from oracle.spectral import PadePoissonPeriodic, PadePoissonWalls, Spectral
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(32, 16, 32), (16, 16, 256), (64, 32, 16)]
+
+
+def _fields(nx, ny, nz, seed, wall_w=0.0):
+    L = 2 * np.pi
+    dx, dy, dz = L / nx, L / ny, 1.0 / nz
+    rng = np.random.default_rng(seed)
+    u, v = rng.standard_normal((2, nx, ny, nz))
+    w = rng.standard_normal((nx, ny, nz + 1))
+    w[:, :, 0] *= wall_w
+    w[:, :, nz] *= wall_w
+    spC = Spectral(nx, ny, nz, dx, dy, dz)
+    spE = Spectral(nx, ny, nz + 1, dx, dy, dz)
+    return (dx, dy, dz), spC, spC.fft(u), spC.fft(v), spE.fft(w)
+
+
+def _dev(a, cuda, dt):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).ravel(order="F"))).to(device=cuda, dtype=dt)
+
+
+def _host(t, shape):
+    import torch
+    torch.cuda.synchronize()
+    return t.detach().cpu().to(torch.complex128).numpy().reshape(shape, order="F")
+
+
+def _gpu(nx, ny, nz, d, uh, vh, wh, prec, cuda, Lz=None):
+    import torch
+    dx, dy, dz = d
+    sp = P.spectral()
+    sp.init(nx, ny, nz, dx, dy, dz, init_periodicInZ=False)
+    pp = P.padepoisson()
+    assert pp.init(sp, dz, PeriodicInZ=False, computeStokesPressure=Lz is not None, Lz=Lz) == 0
+    ct = torch.complex128 if prec == "f64" else torch.complex64
+    U, V, W = _dev(uh, cuda, ct), _dev(vh, cuda, ct), _dev(wh, cuda, ct)
+    assert pp.PressureProjection(U, V, W, prec=prec) == 0
+    nxh = nx // 2 + 1
+    div = torch.empty(nx * ny * nz, dtype=torch.float64 if prec == "f64" else torch.float32, device=cuda)
+    assert pp.DivergenceCheck(U, V, W, div, prec=prec) == 0
+    return (_host(U, (nxh, ny, nz)), _host(V, (nxh, ny, nz)), _host(W, (nxh, ny, nz + 1)),
+            div.double().cpu().numpy())
+
+
+@pytest.mark.parametrize("shape", CASES, ids=["%dx%dx%d" % c for c in CASES])
+def test_walls_fp64(cuda, shape):
+    nx, ny, nz = shape
+    d, spC, uh, vh, wh = _fields(nx, ny, nz, sum(shape))
+    ref = PadePoissonWalls(spC, *d).projection(uh, vh, wh)
+    got = _gpu(nx, ny, nz, d, uh, vh, wh, "f64", cuda)
+    for k, (g, r) in enumerate(zip(got[:3], ref)):
+        e = rel_l2(g, r)
+        print(shape, "uvw"[k], e)
+        assert e <= 1e-12
+    assert np.abs(got[2][:, :, 0]).max() == 0 and np.abs(got[2][:, :, nz]).max() == 0
+    umax = np.abs(spC.ifft(got[0])).max()
+    print("max |div| / max |u|", np.abs(got[3]).max() / umax)
+    assert np.abs(got[3]).max() <= 1e-11 * umax
+
+
+def test_walls_nonzero_wall_values(cuda):
+    """w given on the walls too (the reference extends it as it is)"""
+    nx, ny, nz = 32, 16, 32
+    d, spC, uh, vh, wh = _fields(nx, ny, nz, 5, wall_w=1.0)
+    ref = PadePoissonWalls(spC, *d).projection(uh, vh, wh)
+    got = _gpu(nx, ny, nz, d, uh, vh, wh, "f64", cuda)
+    for g, r in zip(got[:3], ref):
+        assert rel_l2(g, r) <= 1e-12
+
+
+@pytest.mark.parametrize("prec", ["f32_refine", "f32_store"])
+def test_walls_fp32(cuda, prec):
+    nx, ny, nz = 32, 16, 64
+    d, spC, uh, vh, wh = _fields(nx, ny, nz, 9)
+    g64 = _gpu(nx, ny, nz, d, uh, vh, wh, "f64", cuda)
+    g32 = _gpu(nx, ny, nz, d, uh, vh, wh, prec, cuda)
+    for k in range(3):
+        e = rel_l2(g32[k], g64[k])
+        print(prec, "uvw"[k], e)
+        assert e <= 2e-6
+
+
+STOKES = [(32, 16, 32, 1.0), (64, 32, 16, 2.0), (16, 16, 256, 1.0)]
+
+
+def _stokes_fields(nx, ny, nz, Lz, seed):
+    L = 2 * np.pi
+    dx, dy, dz = L / nx, L / ny, Lz / nz
+    rng = np.random.default_rng(seed)
+    u, v = rng.standard_normal((2, nx, ny, nz))
+    w = rng.standard_normal((nx, ny, nz + 1))
+    spC = Spectral(nx, ny, nz, dx, dy, dz)
+    spE = Spectral(nx, ny, nz + 1, dx, dy, dz)
+    return (dx, dy, dz), spC, spC.fft(u), spC.fft(v), spE.fft(w)
+
+
+@pytest.mark.parametrize("case", STOKES, ids=["%dx%dx%d-Lz%g" % c for c in STOKES])
+def test_stokes_fp64(cuda, case):
+    nx, ny, nz, Lz = case
+    d, spC, uh, vh, wh = _stokes_fields(nx, ny, nz, Lz, nx + nz)
+    ref = PadePoissonWalls(spC, *d, Lz=Lz).projection(uh, vh, wh)
+    got = _gpu(nx, ny, nz, d, uh, vh, wh, "f64", cuda, Lz=Lz)
+    for k, (g, r) in enumerate(zip(got[:3], ref)):
+        e = rel_l2(g, r)
+        print(case, "uvw"[k], e)
+        assert e <= 1e-12
+    assert np.abs(got[2][:, :, 0]).max() == 0 and np.abs(got[2][:, :, nz]).max() == 0
+
+
+@pytest.mark.parametrize("prec", ["f32_refine", "f32_store"])
+def test_stokes_fp32(cuda, prec):
+    nx, ny, nz, Lz = 32, 16, 64, 1.0
+    d, spC, uh, vh, wh = _stokes_fields(nx, ny, nz, Lz, 11)
+    g64 = _gpu(nx, ny, nz, d, uh, vh, wh, "f64", cuda, Lz=Lz)
+    g32 = _gpu(nx, ny, nz, d, uh, vh, wh, prec, cuda, Lz=Lz)
+    for k in range(3):
+        e = rel_l2(g32[k], g64[k])
+        print(prec, "uvw"[k], e)
+        assert e <= 2e-6
+
+
+def _pressure_objects(kind, nx, ny, nz, cuda):
+    L = 2 * np.pi
+    dx, dy = L / nx, L / ny
+    sp = P.spectral()
+    pp = P.padepoisson()
+    if kind == "periodic":
+        dz = L / nz
+        sp.init(nx, ny, nz, dx, dy, dz, init_periodicInZ=True)
+        assert pp.init(sp, dz) == 0
+        spo = Spectral(nx, ny, nz, dx, dy, dz, periodic_z=True)
+        return sp, pp, spo, PadePoissonPeriodic(spo, dx, dy, dz), dz
+    Lz = 1.0
+    dz = Lz / nz
+    sp.init(nx, ny, nz, dx, dy, dz, init_periodicInZ=False)
+    st = kind == "stokes"
+    assert pp.init(sp, dz, PeriodicInZ=False, computeStokesPressure=st, Lz=Lz if st else None) == 0
+    spo = Spectral(nx, ny, nz, dx, dy, dz)
+    return sp, pp, spo, PadePoissonWalls(spo, dx, dy, dz, Lz=Lz if st else None), dz
+
+
+@pytest.mark.parametrize("kind", ["periodic", "walls", "stokes"])
+def test_pressure(cuda, kind):
+    import torch
+    nx, ny, nz = 32, 16, 32
+    sp, pp, spo, ref, dz = _pressure_objects(kind, nx, ny, nz, cuda)
+    rng = np.random.default_rng(21)
+    u, v = rng.standard_normal((2, nx, ny, nz))
+    w = rng.standard_normal((nx, ny, nz + 1))
+    if kind == "periodic":
+        w[:, :, nz] = w[:, :, 0]
+    spE = Spectral(nx, ny, nz + 1, *spo_d(spo, dz))
+    uh, vh, wh = spo.fft(u), spo.fft(v), spE.fft(w)
+    if kind == "periodic":
+        wh = np.concatenate([spo.fft(w[:, :, :nz]), spo.fft(w[:, :, :1])], axis=2)
+    nxh = nx // 2 + 1
+    ct = torch.complex128
+    U, V, W = _dev(uh, cuda, ct), _dev(vh, cuda, ct), _dev(wh, cuda, ct)
+    U0, V0, W0 = U.clone(), V.clone(), W.clone()
+    pr = torch.empty(nx * ny * nz, dtype=torch.float64, device=cuda)
+    assert pp.getPressure(U, V, W, pr, prec="f64") == 0
+    torch.cuda.synchronize()
+    assert torch.equal(U, U0) and torch.equal(V, V0) and torch.equal(W, W0)
+    p0 = pr.cpu().numpy().reshape((nx, ny, nz), order="F")
+    e = rel_l2(p0, ref.get_pressure(uh, vh, wh))
+    print(kind, "getPressure", e)
+    assert e <= 1e-12
+    pr2 = torch.empty_like(pr)
+    assert pp.getPressureAndUpdateRHS(U, V, W, pr2, prec="f64") == 0
+    r = ref.get_pressure_and_update(uh, vh, wh)
+    got = (_host(U, (nxh, ny, nz)), _host(V, (nxh, ny, nz)), _host(W, (nxh, ny, nz + 1)),
+           pr2.cpu().numpy().reshape((nx, ny, nz), order="F"))
+    for k in range(4):
+        e = rel_l2(got[k], r[k])
+        print(kind, "update", "uvwp"[k], e)
+        assert e <= 1e-12
+    # FP32 + refinement of the same calls
+    U, V, W = _dev(uh, cuda, torch.complex64), _dev(vh, cuda, torch.complex64), _dev(wh, cuda, torch.complex64)
+    p32 = torch.empty(nx * ny * nz, dtype=torch.float32, device=cuda)
+    assert pp.getPressure(U, V, W, p32, prec="f32_refine") == 0
+    torch.cuda.synchronize()
+    e = rel_l2(p32.double().cpu().numpy().reshape((nx, ny, nz), order="F"), p0)
+    print(kind, "getPressure f32_refine", e)
+    assert e <= 2e-6
+
+
+def spo_d(spo, dz):
+    nx, ny, _ = spo.shape
+    return 2 * np.pi / nx, 2 * np.pi / ny, dz
