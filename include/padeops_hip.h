@@ -375,9 +375,11 @@ int pdo_igrid_interp_primitive(pdo_duidxj h, int prec, const pdo_interp_fields *
 /* ---- SGS eddy-viscosity models (src/incompressible/sgs_models) -------------
  * pdo_sgs_kernel: get_Sij_from_duidxj (eddyViscosity.F90:1-21) fused with the
  * model kernel, model 0 Smagorinsky |S| = sqrt(2 S_ij S_ij)
- * (smagorinsky.F90:45-68) or 2 AMD with constants camd_x/y/z
- * (AMD.F90:23-73); duidxj (npts, 9), Sij (npts, 6), nu_sgs (npts); Sij or
- * nu_sgs may be NULL.
+ * (smagorinsky.F90:45-68), 1 sigma s3 (s1 - s2) (s2 - s3) / s1^2 of the
+ * gradient's singular values (sigma.F90:30-100) or 2 AMD with constants
+ * camd_x/y/z (AMD.F90:23-73); duidxj (npts, 9), Sij (npts, 6), nu_sgs
+ * (npts); Sij or nu_sgs may be NULL.  The model constants of init_sigma
+ * ((Csgs deltaLES)^2, sigma.F90:1-21) are the caller's cglobal.
  * pdo_sgs_scale: multiply_by_model_constant (eddyViscosity.F90:74-103):
  * nu *= cplane[k] on plane k of nxy points (device array, e.g. the dynamic
  * procedure's lambda), or cglobal when cplane is NULL.
@@ -416,17 +418,35 @@ int pdo_dynproc_destroy(pdo_dynproc h);
  * nx ny nz, edges: 2 of nx ny (nz+1)), nu_sgs_C and nu_sgs_E. */
 typedef struct pdo_sgsrhs_s *pdo_sgsrhs;
 typedef struct {
-    int model;        /* 0 Smagorinsky, 2 AMD */
+    int model;        /* 0 Smagorinsky, 1 sigma, 2 AMD */
     int explicit_edge;
     double camd_x, camd_y, camd_z;
     double cglobal;
     const double *cmodelC, *cmodelE;
+    int wall_model;   /* 0 none, 1 Moeng (WallModel 1; needs walls in z) */
+    double z0;        /* roughness length of the wall model, 0 < z0 < dz/2 */
 } pdo_sgs_params;
 int pdo_sgsrhs_init(pdo_sgsrhs *h, pdo_spectral spC, pdo_spectral spE, pdo_pade6stagg dz, const pdo_sgs_params *p,
                     int prec);
 int pdo_sgsrhs_compute(pdo_sgsrhs h, int prec, const void *duidxjC, const void *duidxjE, const double *lambda,
                        void *urhs, void *vrhs, void *wrhs, void *stream);
 int pdo_sgsrhs_fields(pdo_sgsrhs h, const void **tauC, const void **tauE, const void **nuC, const void **nuE);
+/* with a wall model: computeWallStress (wallmodel.F90:33-77, Moeng,
+ * unstratified) from the first cell level of u, v (physical, (nx, ny, nz)):
+ * Uspmn the plane mean of sqrt(u^2 + v^2), ustar = Uspmn kappa /
+ * log(dz / 2 / z0) (kappa 0.4), WallMFactor = -ustar^2 / (Uspmn + 1e-13);
+ * embed_WM_stress (:79-86) then sets tau13 = WallMFactor u and
+ * tau23 = WallMFactor v on the bottom edge plane (after the eddy-viscosity
+ * or learned stress), and the divergence follows as in pdo_sgsrhs_compute,
+ * which refuses an object with a wall model.  The reference takes the
+ * stresses through the spectrum of the first level and back, the same values
+ * to rounding.  Its Bou-Zeid model (WallModel 2) filters with a surface
+ * filter that the reference never sets up, so it is not offered.
+ * pdo_sgsrhs_wall_state: {Uspmn, ustar, WallMFactor, Umn} of the last call
+ * (synchronizes). */
+int pdo_sgsrhs_compute_wall(pdo_sgsrhs h, int prec, const void *duidxjC, const void *duidxjE, const double *lambda,
+                            const void *u, const void *v, void *urhs, void *vrhs, void *wrhs, void *stream);
+int pdo_sgsrhs_wall_state(pdo_sgsrhs h, double *state);
 /* replace the eddy-viscosity stress by a learned closure's (pdo_closure_stress
  * on the cell and edge gradients, filter width delta, feature dratio); with
  * fallback the eddy-viscosity stress stays where the closure's dissipation is

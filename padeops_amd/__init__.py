@@ -855,17 +855,21 @@ def interp_PrimitiveVars(op, uhat, vhat, what, whatC, wC, uEhat, uE, vEhat, vE, 
 
 class sgsRHS:
     """sgs_igrid%getRHS_SGS (sgsmod_igrid.F90:204-266) for the eddy-viscosity
-    models: model "smagorinsky" or "amd"; cmodelC / cmodelE: per-plane model
-    constants (sequences of nz / nz+1) or None for cglobal."""
+    models: model "smagorinsky", "sigma" or "amd"; cmodelC / cmodelE:
+    per-plane model constants (sequences of nz / nz+1) or None for cglobal;
+    wall_model "moeng" with roughness z0 (walls in z; compute then needs the
+    physical u, v)."""
 
     def __init__(self):
         self._h = ctypes.c_void_p()
 
     def init(self, spC, spE, dzop, model="smagorinsky", explicit_edge=False, camd=(1.0, 1.0, 1.0), cglobal=1.0,
-             cmodelC=None, cmodelE=None, prec="f32_refine"):
+             cmodelC=None, cmodelE=None, wall_model=None, z0=0.0, prec="f32_refine"):
         self._keep = (spC, spE, dzop)
         p = _lib.SgsParams()
-        p.model = {"smagorinsky": 0, "amd": 2}[model]
+        p.model = SGS_MODELS[model]
+        p.wall_model = {None: 0, "moeng": 1}[wall_model]
+        p.z0 = z0
         p.explicit_edge = int(explicit_edge)
         p.camd_x, p.camd_y, p.camd_z = camd
         p.cglobal = cglobal
@@ -878,11 +882,23 @@ class sgsRHS:
         return _check(lib().pdo_sgsrhs_init(ctypes.byref(self._h), spC._h, spE._h, dzop._h,
                                             ctypes.cast(ctypes.pointer(p), ctypes.c_void_p), PREC[prec]))
 
-    def compute(self, duidxjC, duidxjE, urhs, vrhs, wrhs, lambda_1d=None, prec="f32_refine", stream=None):
-        """urhs, vrhs, wrhs -= the divergence of the SGS stress (in place)."""
+    def compute(self, duidxjC, duidxjE, urhs, vrhs, wrhs, lambda_1d=None, u=None, v=None, prec="f32_refine",
+                stream=None):
+        """urhs, vrhs, wrhs -= the divergence of the SGS stress (in place);
+        u, v: the physical cell velocities, for the wall model."""
+        if u is not None or v is not None:
+            return _check(lib().pdo_sgsrhs_compute_wall(self._h, _prec(duidxjC, prec), _ptr(duidxjC),
+                                                        _ptr(duidxjE), _ptr(lambda_1d), _ptr(u), _ptr(v),
+                                                        _ptr(urhs), _ptr(vrhs), _ptr(wrhs), _stream(stream)))
         return _check(lib().pdo_sgsrhs_compute(self._h, _prec(duidxjC, prec), _ptr(duidxjC), _ptr(duidxjE),
                                                _ptr(lambda_1d), _ptr(urhs), _ptr(vrhs), _ptr(wrhs),
                                                _stream(stream)))
+
+    def wall_state(self):
+        """(Uspmn, ustar, WallMFactor, Umn) of the last compute"""
+        out = (ctypes.c_double * 4)()
+        _check(lib().pdo_sgsrhs_wall_state(self._h, out))
+        return tuple(out)
 
     def fields(self):
         """device pointers (tauC, tauE, nuC, nuE) of the last compute"""
@@ -923,7 +939,7 @@ class igrid:
         p.Re, p.scheme, p.CFL, p.dt = Re, scheme, CFL, dt
         if sgs is not None:
             p.use_sgs = 1
-            p.sgs.model = {"smagorinsky": 0, "amd": 2}[sgs.get("model", "smagorinsky")]
+            p.sgs.model = SGS_MODELS[sgs.get("model", "smagorinsky")]
             p.sgs.explicit_edge = int(sgs.get("explicit_edge", False))
             p.sgs.camd_x, p.sgs.camd_y, p.sgs.camd_z = sgs.get("camd", (1.0, 1.0, 1.0))
             p.sgs.cglobal = sgs.get("cglobal", 1.0)
@@ -1059,10 +1075,14 @@ class _DeviceView:
         self.__cuda_array_interface__ = {"shape": (nbytes,), "typestr": "|u1", "data": (ptr, False), "version": 3}
 
 
+SGS_MODELS = {"smagorinsky": 0, "sigma": 1, "amd": 2}
+
+
 def sgs_kernel(duidxj, Sij, nu_sgs, npts, model="smagorinsky", camd=(0.0, 0.0, 0.0), prec="f32_refine",
                stream=None):
-    """get_Sij_from_duidxj + get_smagorinsky_kernel / get_amd_kernel (sgs_models)."""
-    m = {"smagorinsky": 0, "amd": 2}[model]
+    """get_Sij_from_duidxj + get_smagorinsky_kernel / get_sigma_kernel /
+    get_amd_kernel (sgs_models)."""
+    m = SGS_MODELS[model]
     for t, k in ((duidxj, 9), (Sij, 6), (nu_sgs, 1)):
         if t is not None and t.numel() < k * npts:
             raise ValueError("buffer smaller than the declared shape")

@@ -112,6 +112,8 @@ SIGNATURES = {
     "pdo_dynproc_destroy": ([_vp], _c_int),
     "pdo_igrid_interp_primitive": ([_vp, _c_int, _vp, _vp], _c_int),
     "pdo_sgsrhs_init": ([_pvp, _vp, _vp, _vp, _vp, _c_int], _c_int),
+    "pdo_sgsrhs_compute_wall": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "pdo_sgsrhs_wall_state": ([_vp, _vp], _c_int),
     "pdo_sgsrhs_compute": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "pdo_sgsrhs_fields": ([_vp, _pvp, _pvp, _pvp, _pvp], _c_int),
     "pdo_sgsrhs_set_closure": ([_vp, _vp, _c_double, _c_double, _c_int], _c_int),
@@ -190,7 +192,8 @@ class SgsParams(ctypes.Structure):
     """pdo_sgs_params of include/padeops_hip.h"""
     _fields_ = [("model", ctypes.c_int), ("explicit_edge", ctypes.c_int), ("camd_x", ctypes.c_double),
                 ("camd_y", ctypes.c_double), ("camd_z", ctypes.c_double), ("cglobal", ctypes.c_double),
-                ("cmodelC", ctypes.POINTER(ctypes.c_double)), ("cmodelE", ctypes.POINTER(ctypes.c_double))]
+                ("cmodelC", ctypes.POINTER(ctypes.c_double)), ("cmodelE", ctypes.POINTER(ctypes.c_double)),
+                ("wall_model", ctypes.c_int), ("z0", ctypes.c_double)]
 
 
 class IgridParams(ctypes.Structure):

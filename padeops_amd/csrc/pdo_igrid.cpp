@@ -4,6 +4,7 @@
 // objects (x, y: i k and 2D inverse transforms) and Pade6stagg (z).  On one
 // GPU the reference's y <-> z transposes are identities, so the z operators
 // work on the spectral arrays in place of transposed copies.
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -187,6 +188,9 @@ struct SgsRhsObject {
     pdo_closure closure = nullptr;
     double cl_delta = 0, cl_dratio = 1;
     int cl_fallback = 0;
+    // Moeng wall model: block sums and {Uspmn, ustar, WallMFactor, Umn}
+    double *d_wpart = nullptr, *d_wm = nullptr;
+    static constexpr int WALL_BLOCKS = 256;
     void free_bufs()
     {
         for (void *b : bufs) (void)hipFree(b);
@@ -197,6 +201,8 @@ struct SgsRhsObject {
         free_bufs();
         if (d_cC) (void)hipFree(d_cC);
         if (d_cE) (void)hipFree(d_cE);
+        if (d_wpart) (void)hipFree(d_wpart);
+        if (d_wm) (void)hipFree(d_wm);
     }
     int alloc(int prec)
     {
@@ -226,7 +232,23 @@ int pdo_sgsrhs_init(pdo_sgsrhs *h, pdo_spectral spC, pdo_spectral spE, pdo_pade6
 {
     if (!h || !spC || !spE || !dzop || !p) { set_error("invalid argument"); return PDO_ERR_ARG; }
     *h = nullptr;
-    if (p->model != 0 && p->model != 2) { set_error("eddy-viscosity model: 0 Smagorinsky or 2 AMD"); return PDO_ERR_ARG; }
+    if (p->model < 0 || p->model > 2) {
+        set_error("eddy-viscosity model: 0 Smagorinsky, 1 sigma or 2 AMD");
+        return PDO_ERR_ARG;
+    }
+    if (p->wall_model != 0) {
+        // the Bou-Zeid model (2) filters with spectral%GsurfaceFilter, which
+        // the reference never sets up, so only the Moeng model is offered
+        if (p->wall_model != 1) { set_error("wall model: 0 none or 1 Moeng"); return PDO_ERR_ARG; }
+        if (pade6stagg_periodic(dzop)) {
+            set_error("a wall model needs walls in z (a non-periodic Pade6stagg)");
+            return PDO_ERR_ARG;
+        }
+        if (!(p->z0 > 0) || !(spC->dz > 2 * p->z0)) {
+            set_error("wall model: need 0 < z0 < dz / 2");
+            return PDO_ERR_ARG;
+        }
+    }
     if (spE->nx != spC->nx || spE->ny != spC->ny || spE->nz != spC->nz + 1) {
         set_error("sgs: the edge spectral object must have nz+1 planes of the cell object's grid");
         return PDO_ERR_ARG;
@@ -247,16 +269,24 @@ int pdo_sgsrhs_init(pdo_sgsrhs *h, pdo_spectral spC, pdo_spectral spE, pdo_pade6
         e = hipMalloc(&o->d_cE, (nz + 1) * sizeof(double));
         if (e == hipSuccess) e = hipMemcpy(o->d_cE, p->cmodelE, (nz + 1) * sizeof(double), hipMemcpyHostToDevice);
     }
+    if (e == hipSuccess && p->wall_model) {
+        e = hipMalloc(&o->d_wpart, 2 * SgsRhsObject::WALL_BLOCKS * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&o->d_wm, 4 * sizeof(double));
+    }
     if (e != hipSuccess) { delete o; return hip_fail(e, "sgs model constants"); }
     if (int st = o->alloc(prec)) { delete o; return st; }
     *h = o;
     return 0;
 }
 
-int pdo_sgsrhs_compute(pdo_sgsrhs h, int prec, const void *duidxjC, const void *duidxjE, const double *lambda,
-                       void *urhs, void *vrhs, void *wrhs, void *stream)
+static int sgsrhs_compute(pdo_sgsrhs h, int prec, const void *duidxjC, const void *duidxjE, const double *lambda,
+                          const void *u, const void *v, void *urhs, void *vrhs, void *wrhs, void *stream)
 {
     if (!h || !duidxjC || !duidxjE || !urhs || !vrhs || !wrhs) { set_error("invalid argument"); return PDO_ERR_ARG; }
+    if (h->p.wall_model && (!u || !v)) {
+        set_error("sgs: the wall model needs the first-level velocity (pdo_sgsrhs_compute_wall)");
+        return PDO_ERR_ARG;
+    }
     const int mode = fft_mode(prec);
     if (mode < 0) { set_error("unknown precision mode"); return PDO_ERR_ARG; }
     if (h->wprec != prec) {
@@ -309,6 +339,12 @@ int pdo_sgsrhs_compute(pdo_sgsrhs h, int prec, const void *duidxjC, const void *
         PDO_TRY(pdo_closure_stress(h->closure, prec, duidxjE, (long)h->nE, h->cl_delta, h->cl_dratio, 1,
                                    h->cl_fallback, 0, h->tauE, stream));
     }
+    // computeWallStress and embed_WM_stress (sgsmod_igrid.F90:164, 197): the
+    // wall stresses on the bottom edge plane of tau13, tau23
+    if (p.wall_model)
+        PDO_HIP(launch_wall_model(mode, u, v, h->tauE, nxy, (long)h->nE, h->d_wpart, SgsRhsObject::WALL_BLOCKS, 0.40,
+                                  std::log(C.dz / 2.0 / p.z0), h->d_wm, st),
+                "wall model");
     const size_t W = prec == PDO_F64 ? 8 : 4;
     auto tc = [&](int q) { return (const void *)((const char *)h->tauC + (size_t)q * h->nC * W); };
     auto te = [&](int q) { return (const void *)((const char *)h->tauE + (size_t)q * h->nE * W); };
@@ -336,6 +372,26 @@ int pdo_sgsrhs_compute(pdo_sgsrhs h, int prec, const void *duidxjC, const void *
 #undef PDO_TRY
 #undef PDO_HIP
     return 0;
+}
+
+int pdo_sgsrhs_compute(pdo_sgsrhs h, int prec, const void *duidxjC, const void *duidxjE, const double *lambda,
+                       void *urhs, void *vrhs, void *wrhs, void *stream)
+{
+    return sgsrhs_compute(h, prec, duidxjC, duidxjE, lambda, nullptr, nullptr, urhs, vrhs, wrhs, stream);
+}
+
+int pdo_sgsrhs_compute_wall(pdo_sgsrhs h, int prec, const void *duidxjC, const void *duidxjE, const double *lambda,
+                            const void *u, const void *v, void *urhs, void *vrhs, void *wrhs, void *stream)
+{
+    return sgsrhs_compute(h, prec, duidxjC, duidxjE, lambda, u, v, urhs, vrhs, wrhs, stream);
+}
+
+int pdo_sgsrhs_wall_state(pdo_sgsrhs h, double *state)
+{
+    if (!h || !state) { set_error("invalid argument"); return PDO_ERR_ARG; }
+    if (!h->d_wm) { set_error("sgs: no wall model"); return PDO_ERR_ARG; }
+    hipError_t e = hipMemcpy(state, h->d_wm, 4 * sizeof(double), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? 0 : hip_fail(e, "wall state");
 }
 
 int pdo_sgsrhs_set_closure(pdo_sgsrhs h, pdo_closure c, double delta, double dratio, int fallback)

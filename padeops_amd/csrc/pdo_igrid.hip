@@ -320,6 +320,99 @@ hipError_t launch_tau(int mode, const void *nuC, const void *SC, const void *nuE
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Moeng wall model: first-plane sums, the surface quantities, the embedded
+// wall stresses
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename TS>
+__global__ void __launch_bounds__(256) k_wall_sums(const TS *u, const TS *v, long nxy, double *part)
+{
+    __shared__ double ss[256], su[256];
+    double a = 0, b = 0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < nxy; i += (long)gridDim.x * 256) {
+        const double x = (double)u[i], y = (double)v[i];
+        a += sqrt(x * x + y * y);
+        b += x;
+    }
+    ss[threadIdx.x] = a;
+    su[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ss[threadIdx.x] += ss[threadIdx.x + w];
+            su[threadIdx.x] += su[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = ss[0];
+        part[2 * blockIdx.x + 1] = su[0];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_wall_finalize(const double *part, int nb, double meanfact, double kappa,
+                                                      double lg, double *wm)
+{
+    __shared__ double ss[256], su[256];
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    ss[threadIdx.x] = a;
+    su[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ss[threadIdx.x] += ss[threadIdx.x + w];
+            su[threadIdx.x] += su[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double Uspmn = ss[0] * meanfact;
+        const double ustar = Uspmn * kappa / lg;
+        wm[0] = Uspmn;
+        wm[1] = ustar;
+        wm[2] = -ustar * ustar / (Uspmn + 1.0e-13);
+        wm[3] = su[0] * meanfact;
+    }
+}
+
+template <typename TS>
+__global__ void __launch_bounds__(256) k_wall_embed(const TS *u, const TS *v, TS *tauE, long nxy, long nE,
+                                                   const double *wm)
+{
+    const double M = wm[2];
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < nxy; i += (long)gridDim.x * 256) {
+        tauE[i] = (TS)(M * (double)u[i]);
+        tauE[nE + i] = (TS)(M * (double)v[i]);
+    }
+}
+
+template <typename TS>
+hipError_t wall_t(const void *u, const void *v, void *tauE, long nxy, long nE, double *part, int nb, double kappa,
+                  double lg, double *wm, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_wall_sums<TS>), dim3(nb), dim3(256), 0, st, (const TS *)u, (const TS *)v, nxy, part);
+    hipLaunchKernelGGL(k_wall_finalize, dim3(1), dim3(256), 0, st, (const double *)part, nb, 1.0 / (double)nxy,
+                       kappa, lg, wm);
+    hipLaunchKernelGGL((k_wall_embed<TS>), dim3(blocks_for(nxy)), dim3(256), 0, st, (const TS *)u, (const TS *)v,
+                       (TS *)tauE, nxy, nE, (const double *)wm);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_wall_model(int mode, const void *u, const void *v, void *tauE, long nxy, long nE, double *part,
+                             int nb, double kappa, double lg, double *wm, hipStream_t st)
+{
+    if (mode == 0) return wall_t<double>(u, v, tauE, nxy, nE, part, nb, kappa, lg, wm, st);
+    return wall_t<float>(u, v, tauE, nxy, nE, part, nb, kappa, lg, wm, st);
+}
+
 hipError_t launch_clip_negative(int mode, void *a, long n, hipStream_t st)
 {
     if (mode == 0)

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/padeops_hip.h"
+
 namespace pdo {
 
 // plane 0 of a spectrum of nplane complex values per plane: zero, except the
@@ -32,6 +34,18 @@ hipError_t launch_lincomb(int mode, void *out, const void *const *x, const doubl
 // partial[b] = max over block b of |u|/dx + |v|/dy + |w|/dz (nblocks blocks)
 hipError_t launch_cfl(int mode, const void *u, const void *v, const void *w, long n, double idx, double idy,
                       double idz, double *partial, int nblocks, hipStream_t st);
+
+// Moeng wall model (wallmodel.F90:33-86, 106-134, 162-199 unstratified):
+// sums of sqrt(u^2 + v^2) and u over the first cell plane (nxy points of u,
+// v) in nb blocks, then wm = {Uspmn, ustar, WallMFactor, Umn} with
+// Uspmn = sum / nxy, ustar = Uspmn kappa / lg (lg = log(dz / 2 / z0)),
+// WallMFactor = -ustar^2 / (Uspmn + 1e-13), and on the bottom edge plane of
+// tauE (13 at 0, 23 at nE) tau13 = WallMFactor u, tau23 = WallMFactor v.
+// part holds 2 nb doubles; the sums run in a fixed order.
+hipError_t launch_wall_model(int mode, const void *u, const void *v, void *tauE, long nxy, long nE, double *part,
+                             int nb, double kappa, double lg, double *wm, hipStream_t st);
+// whether a Pade6stagg object is periodic (pdo_pade6.cpp)
+bool pade6stagg_periodic(pdo_pade6stagg h);
 
 struct HitForcingArgs {
     int nxh, ny, nz, nwaves;

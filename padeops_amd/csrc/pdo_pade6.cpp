@@ -8,6 +8,7 @@
 #include <memory>
 
 #include "../../include/padeops_hip.h"
+#include "pdo_igrid.hpp"
 #include "pdo_kernels.hpp"
 #include "pdo_ops.hpp"
 #include "pdo_system.hpp"
@@ -33,6 +34,10 @@ struct Pade6Object {
 }  // namespace pdo
 
 struct pdo_pade6stagg_s : public pdo::Pade6Object {};
+
+namespace pdo {
+bool pade6stagg_periodic(pdo_pade6stagg h) { return h && h->periodic; }
+}  // namespace pdo
 
 using namespace pdo;
 

@@ -38,7 +38,7 @@ extern "C" {
 int pdo_sgs_kernel(int prec, int model, const void *duidxj, void *Sij, void *nu_sgs, long npts, double camd_x,
                    double camd_y, double camd_z, void *stream)
 {
-    if (!duidxj || npts < 0 || (model != 0 && model != 2) || (!Sij && !nu_sgs)) {
+    if (!duidxj || npts < 0 || model < 0 || model > 2 || (!Sij && !nu_sgs)) {
         set_error("invalid argument");
         return PDO_ERR_ARG;
     }

@@ -1,7 +1,8 @@
 // Subgrid-scale eddy-viscosity kernels of igrid's sgs_igrid
 // (src/incompressible/sgs_models): the strain rate of the velocity gradient
 // (eddyViscosity.F90:1-21), the Smagorinsky kernel |S| = sqrt(2 S_ij S_ij)
-// (smagorinsky.F90:45-68), the AMD kernel (AMD.F90:23-73), the model
+// (smagorinsky.F90:45-68), the sigma kernel (sigma.F90:30-100), the AMD
+// kernel (AMD.F90:23-73), the model
 // constant (eddyViscosity.F90:74-103), and the pointwise steps and planar
 // averages of the standard dynamic procedure (standardDynamicProcedure.F90:
 // 74-250).  Fields are (npts) points per component, components `stride`
@@ -20,7 +21,7 @@ static unsigned grid_of(long n)
 }
 
 // S_ij from duidxj (eddyViscosity.F90:8-16), then the model kernel:
-// model 0 Smagorinsky |S|, model 2 AMD (camd_x, y, z); nu may be null
+// model 0 Smagorinsky |S|, 1 sigma, 2 AMD (camd_x, y, z); nu may be null
 template <typename TS, typename TC>
 __global__ void __launch_bounds__(256) k_sgs_kernel(const TS *__restrict__ g, TS *__restrict__ S,
                                                    TS *__restrict__ nu, long n, long stride, int model, TC cx, TC cy,
@@ -50,6 +51,46 @@ __global__ void __launch_bounds__(256) k_sgs_kernel(const TS *__restrict__ g, TS
             q += (TC)2 * (s[4] * s[4]);
             q += s[5] * s[5];
             nu[t] = (TS)sqrt((TC)2 * q);
+        } else if (model == 1) {
+            // sigma.F90:46-96: singular values of the gradient from the
+            // invariants of G = A^T A, nu = s3 (s1 - s2) (s2 - s3) / s1^2,
+            // in FP64 for every storage type: the closed-form cubic roots
+            // lose about half their digits to cancellation
+            using TD = double;
+            TD e[9];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) e[c] = (TD)d[c];
+            const TD G11 = e[0] * e[0] + e[3] * e[3] + e[6] * e[6];
+            const TD G12 = e[0] * e[1] + e[3] * e[4] + e[6] * e[7];
+            const TD G13 = e[0] * e[2] + e[3] * e[5] + e[6] * e[8];
+            const TD G22 = e[1] * e[1] + e[4] * e[4] + e[7] * e[7];
+            const TD G23 = e[1] * e[2] + e[4] * e[5] + e[7] * e[8];
+            const TD G33 = e[2] * e[2] + e[5] * e[5] + e[8] * e[8];
+            const TD I1 = G11 + G22 + G33, I1sq = I1 * I1, I1cu = I1sq * I1;
+            TD I2 = -G11 * G11 - G22 * G22 - G33 * G33;
+            I2 = I2 - (TD)2 * G12 * G12 - (TD)2 * G13 * G13;
+            I2 = I2 - (TD)2 * G23 * G23;
+            I2 = (TD)0.5 * (I2 + I1sq);
+            TD I3 = G11 * (G22 * G33 - G23 * G23);
+            I3 = I3 + G12 * (G13 * G23 - G12 * G33);
+            I3 = I3 + G13 * (G12 * G23 - G22 * G13);
+            TD a1 = I1sq / (TD)9 - I2 / (TD)3;
+            a1 = a1 > (TD)0 ? a1 : (TD)0;
+            const TD a2 = I1cu / (TD)27 - I1 * I2 / (TD)6 + I3 / (TD)2;
+            const TD a1s = sqrt(a1);
+            TD ca = a2 / (a1 * a1s + (TD)1e-13);
+            ca = ca < (TD)1 ? ca : (TD)1;
+            ca = ca > (TD)-1 ? ca : (TD)-1;
+            const TD a3 = acos(ca) / (TD)3;
+            constexpr TD pi3 = 1.04719755119659774615;
+            TD s1sq = I1 / (TD)3 + (TD)2 * a1s * cos(a3);
+            s1sq = s1sq > (TD)0 ? s1sq : (TD)0;
+            const TD s1 = sqrt(s1sq);
+            TD s2 = (TD)-2 * a1s * cos(pi3 + a3) + I1 / (TD)3;
+            s2 = sqrt(s2 > (TD)0 ? s2 : (TD)0);
+            TD s3 = (TD)-2 * a1s * cos(pi3 - a3) + I1 / (TD)3;
+            s3 = sqrt(s3 > (TD)0 ? s3 : (TD)0);
+            nu[t] = (TS)(s3 * (s1 - s2) * (s2 - s3) / (s1sq + (TD)1e-15));
         } else {
             // AMD.F90:43-70: -(C grad u_i . C grad u_j) S_ij / (grad u : grad u)
             auto row = [&](int i, int j) {

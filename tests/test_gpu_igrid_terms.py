@@ -1,7 +1,8 @@
 """GPU parity of igrid%interp_PrimitiveVars (igrid.F90:1423-1463) and of
-sgs_igrid%getRHS_SGS (sgsmod_igrid.F90:204-266, with getTauSGS :156-201)
-against the oracle compositions (oracle/spectral.py interp_primitive,
-oracle/sgs.py rhs_sgs), wall-bounded and periodic in z.
+sgs_igrid%getRHS_SGS (sgsmod_igrid.F90:204-266, with getTauSGS :156-201;
+Smagorinsky, sigma and AMD kernels, and the Moeng wall model) against the
+oracle compositions (oracle/spectral.py interp_primitive, oracle/sgs.py
+rhs_sgs), wall-bounded and periodic in z.
 
 nz = 32 cells: the 33-row edge systems run on the register path with an
 extra row.  FP64 relative L2 <= 1e-12 per output; the FP32+refine path
@@ -12,6 +13,7 @@ import pytest
 import padeops_amd as P
 from fields import rel_l2
 from oracle.sgs import rhs_sgs
+from oracle.sgs import wall_model as G_wall
 from oracle.spectral import Spectral, interp_primitive
 
 pytestmark = pytest.mark.gpu
@@ -97,7 +99,8 @@ def test_interp_primitive(cuda, periodic, prec, bc, temp):
 
 # (model, explicit_edge, model constants: "global" / "planar" / "dynamic")
 SGS_CASES = [("smagorinsky", False, "global"), ("smagorinsky", True, "planar"), ("amd", False, "planar"),
-             ("amd", True, "global"), ("smagorinsky", False, "dynamic")]
+             ("amd", True, "global"), ("smagorinsky", False, "dynamic"), ("sigma", False, "global"),
+             ("sigma", True, "planar")]
 
 
 @pytest.mark.parametrize("periodic", [False, True])
@@ -119,7 +122,7 @@ def test_sgs_rhs(cuda, periodic, prec, case):
     cmE = 0.02 + 0.01 * rng.random(NZ + 1)
     lam = 0.05 + 0.1 * rng.random(NZ)
     camd = (0.3, 0.2, 0.25)
-    kw = dict(model=0 if model == "smagorinsky" else 2, explicit_edge=explicit_edge, camd=camd)
+    kw = dict(model=P.SGS_MODELS[model], explicit_edge=explicit_edge, camd=camd)
     if consts == "global":
         kw["cglobal"] = 0.017
     elif consts == "planar":
@@ -140,7 +143,7 @@ def test_sgs_rhs(cuda, periodic, prec, case):
     lam_d = torch.from_numpy(lam).to(cuda) if consts == "dynamic" else None
     assert obj.compute(d["gC"], d["gE"], d["urhs"], d["vrhs"], d["wrhs"], lambda_1d=lam_d, prec=prec) == 0
     torch.cuda.synchronize()
-    bar = 1e-12 if not single else 5e-6
+    bar = (1e-12 if model != "sigma" else 1e-10) if not single else 5e-6
     for name, t, r, s in (("urhs", d["urhs"], ur, (NXH, NY, NZ)), ("vrhs", d["vrhs"], vr, (NXH, NY, NZ)),
                           ("wrhs", d["wrhs"], wr, (NXH, NY, NZ + 1))):
         err = rel_l2(_host(t, s), r)
@@ -148,3 +151,65 @@ def test_sgs_rhs(cuda, periodic, prec, case):
         assert err <= bar, name
     # the stored stress and eddy viscosities are exposed for inspection
     assert all(p for p in obj.fields())
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32_refine"])
+@pytest.mark.parametrize("closure", ["smagorinsky", "sigma"])
+def test_sgs_rhs_wall_model(cuda, prec, closure):
+    """getRHS_SGS with the Moeng wall model on the wall-bounded grid: the
+    first-level means and u* against the oracle, the bottom-edge stresses
+    WallMFactor u, v (embedded after the eddy-viscosity stress), and the
+    right-hand sides; an object with a wall model refuses the plain compute,
+    and a periodic Pade6stagg refuses the wall model."""
+    import torch
+    single = prec != "f64"
+    spC, spE = Spectral(NX, NY, NZ, DX, DY, DZ), Spectral(NX, NY, NZ + 1, DX, DY, DZ)
+    rng = np.random.default_rng(31)
+    cf = lambda planes: rng.standard_normal((NXH, NY, planes)) + 1j * rng.standard_normal((NXH, NY, planes))  # noqa
+    u = 4.0 + 0.5 * rng.standard_normal((NX, NY, NZ))
+    v = -1.0 + 0.5 * rng.standard_normal((NX, NY, NZ))
+    ins = dict(gC=rng.standard_normal((NX, NY, NZ, 9)), gE=rng.standard_normal((NX, NY, NZ + 1, 9)), urhs=cf(NZ),
+               vrhs=cf(NZ), wrhs=cf(NZ + 1), u=u, v=v)
+    if single:
+        ins = {k: _r32(val) for k, val in ins.items()}
+    z0 = 1e-4
+    uh, vh = spC.fft(ins["u"]), spC.fft(ins["v"])
+    ur, vr, wr, tau, _, _ = rhs_sgs(spC, spE, DZ, False, ins["gC"], ins["gE"], ins["urhs"], ins["vrhs"], ins["wrhs"],
+                                    model=P.SGS_MODELS[closure], cglobal=0.017, wall=(ins["u"], ins["v"], uh, vh, z0))
+    sc, se, d6, _ = _objects(False, BCS[0])
+    obj = P.sgsRHS()
+    assert obj.init(sc, se, d6, model=closure, cglobal=0.017, wall_model="moeng", z0=z0, prec=prec) == 0
+    d = {k: _dev(val, cuda, single) for k, val in ins.items()}
+    with pytest.raises(P.PdoError):
+        obj.compute(d["gC"], d["gE"], d["urhs"], d["vrhs"], d["wrhs"], prec=prec)
+    assert obj.compute(d["gC"], d["gE"], d["urhs"], d["vrhs"], d["wrhs"], u=d["u"], v=d["v"], prec=prec) == 0
+    torch.cuda.synchronize()
+    ref = G_wall(spC, ins["u"], ins["v"], uh, DZ, z0)
+    got = obj.wall_state()
+    print("wall state", got, ref)
+    for g, r in zip(got, ref):
+        assert abs(g - r) <= 1e-12 * abs(r)
+    bar = (1e-12 if closure != "sigma" else 1e-10) if not single else 5e-6
+    for name, t, r, s in (("urhs", d["urhs"], ur, (NXH, NY, NZ)), ("vrhs", d["vrhs"], vr, (NXH, NY, NZ)),
+                          ("wrhs", d["wrhs"], wr, (NXH, NY, NZ + 1))):
+        err = rel_l2(_host(t, s), r)
+        print(name, closure, prec, err)
+        assert err <= bar, name
+    # the bottom edge planes hold the wall stresses
+    import ctypes
+    W = 8 if not single else 4
+    nE = NX * NY * (NZ + 1)
+    tauE = (ctypes.c_double if not single else ctypes.c_float) * (2 * nE)
+    host = tauE()
+    assert P._lib.lib().pdo_memcpy_d2h(ctypes.cast(host, ctypes.c_void_p), ctypes.c_void_p(obj.fields()[1]),
+                                       2 * nE * W) == 0
+    te = np.frombuffer(host, dtype=np.float64 if not single else np.float32).astype(np.float64)
+    t13 = te[:nE].reshape((NX, NY, NZ + 1), order="F")[:, :, 0]
+    t23 = te[nE:].reshape((NX, NY, NZ + 1), order="F")[:, :, 0]
+    sb = 1e-12 if not single else 1e-6
+    assert np.abs(t13 - tau["13"][:, :, 0]).max() <= sb * np.abs(tau["13"][:, :, 0]).max()
+    assert np.abs(t23 - tau["23"][:, :, 0]).max() <= sb * np.abs(tau["23"][:, :, 0]).max()
+    # a wall model needs walls
+    sc2, se2, d62, _ = _objects(True, BCS[0])
+    with pytest.raises(P.PdoError):
+        P.sgsRHS().init(sc2, se2, d62, model=closure, wall_model="moeng", z0=z0, prec=prec)

@@ -1,7 +1,9 @@
-"""GPU parity of the SGS kernels (Smagorinsky, AMD, model constants) and of
-the standard dynamic procedure against oracle/sgs.py.  Pointwise kernels:
-FP64 relative L2 <= 1e-14; the dynamic coefficient (planar sums): <= 1e-10;
-FP32 storage: <= 1e-6."""
+"""GPU parity of the SGS kernels (Smagorinsky, sigma, AMD, model constants)
+and of the standard dynamic procedure against oracle/sgs.py.  Pointwise
+kernels: FP64 relative L2 <= 1e-14 (sigma <= 1e-10: its closed-form cubic
+roots pass through acos and cos, whose last bits differ between libraries and
+are amplified where two singular values nearly coincide); the dynamic
+coefficient (planar sums): <= 1e-10; FP32 storage: <= 1e-6."""
 import numpy as np
 import pytest
 
@@ -13,7 +15,7 @@ from oracle.spectral import Spectral
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("model", ["smagorinsky", "amd"])
+@pytest.mark.parametrize("model", ["smagorinsky", "sigma", "amd"])
 @pytest.mark.parametrize("prec", ["f64", "f32_store"])
 def test_sgs_kernel(cuda, model, prec):
     import torch
@@ -29,13 +31,17 @@ def test_sgs_kernel(cuda, model, prec):
     nu = torch.empty(npts, dtype=dt, device=cuda)
     assert P.sgs_kernel(gd, S, nu, npts, model, camd, prec=prec) == 0
     Sref = G.sij(g)
-    nref = G.smagorinsky(Sref) if model == "smagorinsky" else G.amd(g, Sref, *camd)
+    nref = {"smagorinsky": lambda: G.smagorinsky(Sref), "sigma": lambda: G.sigma(g),
+            "amd": lambda: G.amd(g, Sref, *camd)}[model]()
     bar = 1e-14 if prec == "f64" else 1e-6
     assert rel_l2(from_dev(S, shape + (6,)), Sref) <= bar
-    assert rel_l2(from_dev(nu, shape), nref) <= bar * 10
+    nbar = 1e-10 if (model == "sigma" and prec == "f64") else bar * 10
+    e = rel_l2(from_dev(nu, shape), nref)
+    print(model, prec, "nu", e)
+    assert e <= nbar
     cpl = torch.tensor(np.linspace(0.5, 1.5, shape[2]), dtype=torch.float64, device=cuda)
     assert P.sgs_scale(nu, shape[0] * shape[1], shape[2], cpl, prec=prec) == 0
-    assert rel_l2(from_dev(nu, shape), nref * np.linspace(0.5, 1.5, shape[2])[None, None, :]) <= bar * 10
+    assert rel_l2(from_dev(nu, shape), nref * np.linspace(0.5, 1.5, shape[2])[None, None, :]) <= nbar
     assert P.sgs_scale(nu, shape[0] * shape[1], shape[2], None, 2.0, prec=prec) == 0
 
 

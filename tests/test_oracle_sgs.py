@@ -1,7 +1,9 @@
 """Pins of the SGS oracle (oracle/sgs.py): closed forms of the Smagorinsky and
 AMD kernels, an independent tensor formulation of both, and invariances of
 the dynamic procedure (an identity test filter gives no resolved stress; the
-coefficient does not depend on the velocity scale)."""
+coefficient does not depend on the velocity scale).  The sigma model against
+numpy's singular values and the flows where it must vanish; the Moeng wall
+stress against the log law of a uniform first-level velocity."""
 import numpy as np
 
 import oracle.sgs as G
@@ -116,3 +118,54 @@ def test_dynamic_coefficient_closed_form():
     # the opposite strain gives <L:M> < 0 in every plane: clipped to zero
     lam = G.dynamic_procedure(sp, dx, dx, 2.0, u, v, w, sp.fft(u), sp.fft(v), sp.fft(w), -S, Dsgs)
     assert np.all(lam == 0.0)
+
+
+def test_sigma_is_nicouds_model_of_the_singular_values():
+    """The closed-form roots equal numpy's singular values of A: for random
+    gradients nu = s3 (s1 - s2) (s2 - s3) / s1^2 with s = svd(A)."""
+    rng = np.random.default_rng(21)
+    A = rng.standard_normal((500, 3, 3)) * rng.uniform(0.1, 5.0, (500, 1, 1))
+    s = np.linalg.svd(A, compute_uv=False)
+    ref = s[:, 2] * (s[:, 0] - s[:, 1]) * (s[:, 1] - s[:, 2]) / (s[:, 0] ** 2)
+    got = G.sigma(A.reshape(500, 9))
+    scale = np.abs(A).reshape(500, -1).max(axis=1)
+    assert np.abs(got - ref).max() <= 1e-9 * scale.max()
+    assert (got >= 0).all()
+
+
+def test_sigma_vanishes_where_nicoud_requires():
+    """Two-dimensional flow (s3 = 0), solid rotation (s1 = s2) and isotropic
+    expansion (all equal) give no eddy viscosity; a pure shear is 2D."""
+    rng = np.random.default_rng(22)
+    twod = np.zeros((50, 3, 3))
+    twod[:, :2, :2] = rng.standard_normal((50, 2, 2))
+    w = rng.standard_normal((50, 3))
+    rot = np.zeros((50, 3, 3))
+    rot[:, 0, 1], rot[:, 1, 0] = -w[:, 2], w[:, 2]
+    rot[:, 0, 2], rot[:, 2, 0] = w[:, 1], -w[:, 1]
+    rot[:, 1, 2], rot[:, 2, 1] = -w[:, 0], w[:, 0]
+    iso = np.eye(3)[None] * rng.uniform(0.5, 2.0, (50, 1, 1))
+    for A in (twod, rot, iso):
+        nu = G.sigma(A.reshape(50, 9))
+        assert np.abs(nu).max() <= 1e-6 * np.abs(A).max() ** 1, np.abs(nu).max()
+
+
+def test_moeng_wall_stress_is_the_log_law():
+    """A uniform first-level velocity (U, V) gives
+    ustar = kappa |U| / ln(dz / (2 z0)) and a wall stress of magnitude ustar^2
+    against the velocity: tau13 = -ustar^2 U/|U|, tau23 = -ustar^2 V/|U|."""
+    nx, ny, nz, dz, z0 = 16, 8, 8, 0.05, 1e-4
+    sp = Spectral(nx, ny, nz, 2 * np.pi / nx, 2 * np.pi / ny, dz)
+    U, V = 3.0, -1.2
+    rng = np.random.default_rng(23)
+    u = np.empty((nx, ny, nz))
+    v = np.empty((nx, ny, nz))
+    u[:, :, 0], v[:, :, 0] = U, V
+    u[:, :, 1:] = rng.standard_normal((nx, ny, nz - 1))
+    v[:, :, 1:] = rng.standard_normal((nx, ny, nz - 1))
+    Uspmn, ustar, M, Umn = G.wall_model(sp, u, v, sp.fft(u), dz, z0)
+    mag = np.hypot(U, V)
+    assert abs(Uspmn - mag) <= 1e-13 * mag and abs(Umn - U) <= 1e-13 * U
+    assert abs(ustar - 0.4 * mag / np.log(dz / (2 * z0))) <= 1e-13 * ustar
+    assert abs(M * U + ustar ** 2 * U / mag) <= 1e-12 * ustar ** 2
+    assert abs(M * V + ustar ** 2 * V / mag) <= 1e-12 * ustar ** 2
