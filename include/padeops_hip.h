@@ -648,6 +648,48 @@ int pdo_operators_divergence(pdo_derivatives der, int prec, const void *u, const
                              const int *bc, void *stream);
 int pdo_operators_filter3d(pdo_filters fil, int prec, void *arr, int numtimes, const int *bc, void *stream);
 
+/* ---- cgrid: compressible right-hand side, one species of ideal gas --------
+ * replaces type(cgrid)'s getRHS (src/compressible/cgrid.F90:822-894, without
+ * the problem's source hook) with get_primitive :763-797, getRHS_x/y/z
+ * :896-1056, getLAD :1058-1257 (artificial shear and bulk viscosity and
+ * conductivity, each filtered twice per direction with the Gaussian filter
+ * gfil), get_tau :1360-1404, get_q :1406-1452, and filter :1259-1336.
+ * Ideal gas (IdealGasEOS.F90: p = (gam-1) rho e, T = (gam-1) e / Rgas,
+ * c = sqrt(gam p / rho)); power-law viscosity (PowerLawViscosity.F90:
+ * mu = mu_ref (T/T_ref)^n, bulk 0, kappa = Cp mu / Pr), or none when
+ * inviscid (the reference's default).  One species only: the mixture's
+ * species fluxes and diffusivities are not built.
+ * Sizes (nx, ny, nz) on one GPU; derivative[3] as pdo_derivatives_init
+ * (1 cd10, cgrid's default; 2 cd06), filter[3] the conserved-variable filter
+ * fil as pdo_filters_init (1 cf90, cgrid's default); x_bc, y_bc, z_bc the
+ * reference's (bc1, bcn) per direction (ignored where periodic).
+ * pdo_cgrid_rhs: W (nx, ny, nz, 5) = rho, rho u, rho v, rho w, TE; rhs the
+ * same layout.  pdo_cgrid_filter: cgrid%filter of one field in place,
+ * numtimes passes along y, x, z with fil (gaussian = 0) or gfil.
+ * pdo_cgrid_field (device pointers of the last rhs): 0 rho, 1 u, 2 v, 3 w,
+ * 4 e, 5 p, 6 T, 7 mu, 8 bulk, 9 kap (physical + artificial), 10 duidxj
+ * (9 fields, du_i/dx_j at 3 i + j), 11 tau (xx xy xz yy yz zz), 12 q
+ * (3 fields).  prec at init sizes the work (43 fields); rhs and filter must
+ * use it.  Pointwise arithmetic is FP64 for every storage type. */
+typedef struct pdo_cgrid_s *pdo_cgrid;
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy, dz;
+    int periodic[3];
+    int x_bc[2], y_bc[2], z_bc[2];
+    int derivative[3];
+    int filter[3];
+    double gam, Rgas;
+    double Cmu, Cbeta, Ckap;
+    int inviscid;
+    double mu_ref, T_ref, visc_n, Pr;
+} pdo_cgrid_params;
+int pdo_cgrid_init(pdo_cgrid *h, const pdo_cgrid_params *p, int prec);
+int pdo_cgrid_rhs(pdo_cgrid h, int prec, const void *W, void *rhs, void *stream);
+int pdo_cgrid_filter(pdo_cgrid h, int prec, void *arr, int numtimes, int gaussian, void *stream);
+int pdo_cgrid_field(pdo_cgrid h, int which, const void **ptr);
+int pdo_cgrid_destroy(pdo_cgrid h);
+
 /* ---- full RHS evaluation (the benchmarked composite) ----------------------
  * On a triply periodic n^3 grid of side L, for velocity (u, v, w):
  *   grad(:,:,:,3*i+j) = d u_i / d x_j          9 cd06 first derivatives

@@ -918,6 +918,76 @@ class sgsRHS:
             pass
 
 
+class cgrid:
+    """getRHS of type(cgrid) (src/compressible/cgrid.F90:822-894) for one
+    species of ideal gas, with the artificial diffusivities of getLAD and the
+    cgrid filter.  shape (nx, ny, nz), spacing (dx, dy, dz); periodic,
+    bc ((x_bc1, x_bcn), (y_bc1, y_bcn), (z_bc1, z_bcn)); derivative and
+    filter per direction ("cd10" / "cd06"; "cf90" / "gaussian" / "lstsq");
+    inviscid (the reference's default) or the power-law viscosity mu_ref,
+    T_ref, visc_n, Pr."""
+
+    FIELDS = {"rho": 0, "u": 1, "v": 2, "w": 3, "e": 4, "p": 5, "T": 6, "mu": 7, "bulk": 8, "kap": 9,
+              "duidxj": 10, "tau": 11, "q": 12}
+    _DER = {"cd10": 1, "cd06": 2}
+    _FIL = {"cf90": 1, "gaussian": 2, "lstsq": 3}
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def init(self, shape, spacing, periodic=(True, True, True), bc=((0, 0), (0, 0), (0, 0)),
+             derivative=("cd10",) * 3, filter=("cf90",) * 3, gam=1.4, Rgas=1.0, Cmu=0.002, Cbeta=1.75,
+             Ckap=0.01, inviscid=True, mu_ref=1.0, T_ref=1.0, visc_n=0.75, Pr=0.75, prec="f64"):
+        p = _lib.CgridParams()
+        p.nx, p.ny, p.nz = shape
+        p.dx, p.dy, p.dz = spacing
+        for i in range(3):
+            p.periodic[i] = int(bool(periodic[i]))
+            p.derivative[i] = self._DER[derivative[i]]
+            p.filter[i] = self._FIL[filter[i]]
+        p.x_bc[0], p.x_bc[1] = bc[0]
+        p.y_bc[0], p.y_bc[1] = bc[1]
+        p.z_bc[0], p.z_bc[1] = bc[2]
+        p.gam, p.Rgas, p.Cmu, p.Cbeta, p.Ckap = gam, Rgas, Cmu, Cbeta, Ckap
+        p.inviscid = int(bool(inviscid))
+        p.mu_ref, p.T_ref, p.visc_n, p.Pr = mu_ref, T_ref, visc_n, Pr
+        self.shape, self.prec = tuple(shape), prec
+        return _check(lib().pdo_cgrid_init(ctypes.byref(self._h), ctypes.cast(ctypes.pointer(p), ctypes.c_void_p),
+                                           PREC[prec]))
+
+    def _n(self):
+        return self.shape[0] * self.shape[1] * self.shape[2]
+
+    def getRHS(self, W, rhs, stream=None):
+        """W, rhs: flat tensors of (nx, ny, nz, 5) (rho, rho u, rho v, rho w, TE)."""
+        if _reals(W) < 5 * self._n() or _reals(rhs) < 5 * self._n():
+            raise ValueError("buffer smaller than the declared shape")
+        return _check(lib().pdo_cgrid_rhs(self._h, _prec(W, self.prec), _ptr(W), _ptr(rhs), _stream(stream)))
+
+    def filter(self, arr, numtimes=1, gaussian=False, stream=None):
+        if _reals(arr) < self._n():
+            raise ValueError("buffer smaller than the declared shape")
+        return _check(lib().pdo_cgrid_filter(self._h, _prec(arr, self.prec), _ptr(arr), int(numtimes),
+                                             int(bool(gaussian)), _stream(stream)))
+
+    def field(self, name):
+        """device pointer of a field of the last getRHS"""
+        out = ctypes.c_void_p()
+        _check(lib().pdo_cgrid_field(self._h, self.FIELDS[name], ctypes.byref(out)))
+        return out.value
+
+    def destroy(self):
+        if self._h:
+            lib().pdo_cgrid_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class igrid:
     """Time advancement of type(igrid) (src/incompressible/igrid.F90) on one
     GPU for domains periodic in x, y and z: TVD-RK3 (scheme 1) or SSP-RK45

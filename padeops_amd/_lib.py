@@ -112,6 +112,11 @@ SIGNATURES = {
     "pdo_dynproc_destroy": ([_vp], _c_int),
     "pdo_igrid_interp_primitive": ([_vp, _c_int, _vp, _vp], _c_int),
     "pdo_sgsrhs_init": ([_pvp, _vp, _vp, _vp, _vp, _c_int], _c_int),
+    "pdo_cgrid_init": ([_pvp, _vp, _c_int], _c_int),
+    "pdo_cgrid_rhs": ([_vp, _c_int, _vp, _vp, _vp], _c_int),
+    "pdo_cgrid_filter": ([_vp, _c_int, _vp, _c_int, _c_int, _vp], _c_int),
+    "pdo_cgrid_field": ([_vp, _c_int, _pvp], _c_int),
+    "pdo_cgrid_destroy": ([_vp], _c_int),
     "pdo_sgsrhs_compute_wall": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "pdo_sgsrhs_wall_state": ([_vp, _vp], _c_int),
     "pdo_sgsrhs_compute": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
@@ -194,6 +199,17 @@ class SgsParams(ctypes.Structure):
                 ("camd_y", ctypes.c_double), ("camd_z", ctypes.c_double), ("cglobal", ctypes.c_double),
                 ("cmodelC", ctypes.POINTER(ctypes.c_double)), ("cmodelE", ctypes.POINTER(ctypes.c_double)),
                 ("wall_model", ctypes.c_int), ("z0", ctypes.c_double)]
+
+
+class CgridParams(ctypes.Structure):
+    """pdo_cgrid_params of include/padeops_hip.h"""
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("dx", ctypes.c_double),
+                ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("periodic", ctypes.c_int * 3),
+                ("x_bc", ctypes.c_int * 2), ("y_bc", ctypes.c_int * 2), ("z_bc", ctypes.c_int * 2),
+                ("derivative", ctypes.c_int * 3), ("filter", ctypes.c_int * 3), ("gam", ctypes.c_double),
+                ("Rgas", ctypes.c_double), ("Cmu", ctypes.c_double), ("Cbeta", ctypes.c_double),
+                ("Ckap", ctypes.c_double), ("inviscid", ctypes.c_int), ("mu_ref", ctypes.c_double),
+                ("T_ref", ctypes.c_double), ("visc_n", ctypes.c_double), ("Pr", ctypes.c_double)]
 
 
 class IgridParams(ctypes.Structure):
