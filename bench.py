@@ -142,6 +142,10 @@ def main():
 
     # the wall-bounded 512 x 512 x 256 box: per-operator times and errors
     wall = None if args.no_breakdown else wall_box(P, dev, stream, args.steps)
+    torch.cuda.empty_cache()
+    # the compressible (cgrid) right-hand side at 256^3
+    comp = None if args.no_breakdown else cgrid_bench(P, dev, stream, max(3, args.steps // 4))
+    torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
@@ -177,6 +181,7 @@ def main():
         "roofline": roofline,
         "pass_ms": breakdown,
         "wall_box_512x512x256": wall,
+        "cgrid_rhs_256": comp,
         "hbm_probe": probe,
         "cpu_baseline": cpu,
         "wall_s_timed_fp32r": round(wall32, 4),
@@ -199,6 +204,74 @@ def filter_errors(P, n, f64, f32, stream):
         torch.cuda.synchronize()
         out[name] = rel_l2(b, a)
         del a, b
+    return out
+
+
+# algorithmic words per point of one cgrid RHS evaluation: 69 compact passes
+# (9 velocity gradients; the artificial shear viscosity's 6 second-derivative
+# and 6 Gaussian passes; the bulk viscosity's 3 + 6 + 6 and the conductivity's
+# 3 + 6 + 6; 3 temperature gradients; 15 flux derivatives) of 2 words each,
+# and the pointwise kernels' reads and writes (primitives 12, transport 4,
+# strain magnitude 10, mustar 5, dilatation 4, bulkstar 15, kapstar 10,
+# coefficient sums 9, tau 17, heat flux 9, fluxes 3 x 15, rhs updates
+# 10 + 2 x 15)
+CGRID_WORDS = 69 * 2 + 12 + 4 + 10 + 5 + 4 + 15 + 10 + 9 + 17 + 9 + 45 + 40
+
+
+def cgrid_bench(P, dev, stream, steps):
+    """The compressible RHS of cgrid (one species, viscous, LAD, triply
+    periodic cd10) at 256^3 with a smooth seeded state: ms per evaluation in
+    FP64 and FP32+refine (HIP events on the launch stream), the algorithmic
+    GB/s of CGRID_WORDS words per point, and the FP32+refine error of each
+    RHS component against FP64 on the same FP32-rounded state."""
+    import math
+
+    import torch
+    n = 256
+    L = 2 * math.pi
+    x = torch.arange(n, device=dev, dtype=torch.float64) * (L / n)
+    X, Y, Z = torch.meshgrid(x, x, x, indexing="ij")
+    g = torch.Generator().manual_seed(99)
+
+    def smooth():
+        a = (torch.rand(6, generator=g, dtype=torch.float64) * 2 - 1).tolist()
+        return (a[0] * torch.sin(X + a[1]) * torch.cos(2 * Y) + a[2] * torch.cos(3 * Z + Y)
+                + a[3] * torch.sin(2 * X - Z) + a[4] * torch.cos(Y + a[5]))
+    rho = 1.0 + 0.15 * smooth()
+    u, v, w = 0.3 * smooth(), 0.3 * smooth(), 0.3 * smooth()
+    p = 1.0 + 0.1 * smooth()
+    e = p / (0.4 * rho)
+    # Fortran order (x fastest): permute so that the flat tensor runs x first
+    comps = [rho, rho * u, rho * v, rho * w, rho * (e + 0.5 * (u * u + v * v + w * w))]
+    W = torch.stack([c.permute(2, 1, 0).contiguous() for c in comps]).reshape(-1)
+    del X, Y, Z, rho, u, v, w, p, e, comps
+    W = W.float().double()
+    N = n ** 3
+    out = {"grid": n, "words_per_point": CGRID_WORDS, "compact_passes": 69}
+    res = {}
+    for prec, dt in (("f64", torch.float64), ("f32_refine", torch.float32)):
+        cg = P.cgrid()
+        cg.init((n, n, n), (L / n,) * 3, inviscid=False, mu_ref=0.01, T_ref=1.0, visc_n=0.75, Pr=0.72, prec=prec)
+        Wd = W.to(dt)
+        r = torch.empty_like(Wd)
+        cg.getRHS(Wd, r, stream=stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(steps):
+            cg.getRHS(Wd, r, stream=stream)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        out["ms_" + prec] = round(ms, 4)
+        out["GBps_" + prec] = round(CGRID_WORDS * N * Wd.element_size() / (ms * 1e-3) / 1e9, 1)
+        res[prec] = r.double()
+        del cg, Wd, r
+        torch.cuda.empty_cache()
+    out["speedup_fp32r_vs_fp64"] = round(out["ms_f64"] / out["ms_f32_refine"], 3)
+    out["fp32r_rel_l2_vs_fp64"] = {c: float("%.3e" % rel_l2(res["f32_refine"][c * N:(c + 1) * N],
+                                                            res["f64"][c * N:(c + 1) * N]))
+                                   for c in range(5)}
     return out
 
 
