@@ -455,9 +455,14 @@ int pdo_sgsrhs_wall_state(pdo_sgsrhs h, double *state);
 int pdo_sgsrhs_set_closure(pdo_sgsrhs h, pdo_closure c, double delta, double dratio, int fallback);
 int pdo_sgsrhs_destroy(pdo_sgsrhs h);
 
-/* ---- igrid time advancement, periodic in z ----------------------------------
+/* ---- igrid time advancement, periodic in z or with walls ------------------
  * replaces the time loop of type(igrid) (src/incompressible/igrid.F90) for
- * domains periodic in x, y and z (the forced isotropic turbulence setup):
+ * domains periodic in x, y and z (the forced isotropic turbulence setup) or
+ * with walls in z (PeriodicInZ = .false.: non-periodic spectral objects,
+ * Pade6stagg and PadePoisson, the boundary kinds of botWall / topWall, the
+ * x-y dealias of the spectra, the SGS wall model of a wall-model bottom
+ * wall, the mean pressure gradient dpFdx added to the mean mode of u;
+ * no HIT forcing):
  *   timeAdvance :1056-1076 with scheme 1 TVD_RK3 :1105-1170 or 2 SSP_RK45
  *     :1172-1299 (timeSteppingScheme); the stage-1 projection is taken
  *     explicitly, as with fastCalcPressure = .false.
@@ -491,6 +496,14 @@ typedef struct {
     double kmin, kmax, eps_amplitude;
     int nwaves;
     unsigned long long seed;
+    /* walls in z (0: periodic).  botWall 1 no-slip, 2 slip, 3 wall model
+     * (with sgs.wall_model); topWall 1 or 2 (get_boundary_conditions_stencil
+     * igrid.F90:5148-5220); dpFdx the mean pressure gradient of
+     * addExtraForcingTerm (:1759-1765, 0: none); stokes_pressure the
+     * PadePoisson Stokes pressure (computeStokesPressure, Lz) */
+    int walls, botWall, topWall;
+    double dpFdx;
+    int stokes_pressure;
 } pdo_igrid_params;
 int pdo_igrid_init(pdo_igrid *h, const pdo_igrid_params *p, int prec);
 int pdo_igrid_set_velocity(pdo_igrid h, const void *u, const void *v, const void *w, void *stream);

@@ -1,7 +1,14 @@
 """TEST INFRASTRUCTURE ONLY (the checker): numpy restatement of the time
 advancement of igrid (src/incompressible/igrid.F90) for domains periodic in
-z, composed from the oracle's spectral, Pade6stagg, PadePoisson, duidxj and
-SGS restatements.  Only tests/ use it; the product path never does.
+z or with walls in z, composed from the oracle's spectral, Pade6stagg,
+PadePoisson, duidxj and SGS restatements.  Only tests/ use it; the product
+path never does.
+
+  walls              PeriodicInZ = .false.: the boundary kinds of
+                     get_boundary_conditions_stencil :5148-5220 for botWall /
+                     topWall (1 no-slip, 2 slip, 3 wall model), the x-y
+                     dealias, the PadePoisson of the 2 nz extensions, the SGS
+                     wall model, addExtraForcingTerm :1759-1765 (dpFdx)
 
   project_and_prep   igrid.F90:1961-1992 (dealiasFields :1020-1037)
   populate_rhs       :1793-1912 with AddNonLinearTerm_Rot :1497-1566,
@@ -17,7 +24,7 @@ import numpy as np
 
 from . import STG_IN_E2C
 from .sgs import rhs_sgs
-from .spectral import PadePoissonPeriodic, Spectral, duidxj, interp_primitive, pade6_apply
+from .spectral import PadePoissonPeriodic, PadePoissonWalls, Spectral, duidxj, interp_primitive, pade6_apply
 
 _M64 = (1 << 64) - 1
 
@@ -43,29 +50,44 @@ A52, A53, A54 = 0.517231671970585, 0.096059710526147, 0.386708617503269
 
 
 class Igrid:
-    def __init__(self, nx, ny, nz, Lx, Ly, Lz, Re=0.0, scheme=2, CFL=0.0, dt=0.0, sgs=None, forcing=None):
+    def __init__(self, nx, ny, nz, Lx, Ly, Lz, Re=0.0, scheme=2, CFL=0.0, dt=0.0, sgs=None, forcing=None,
+                 walls=None):
         self.nx, self.ny, self.nz = nx, ny, nz
         self.dx, self.dy, self.dz = Lx / nx, Ly / ny, Lz / nz
         self.Re, self.scheme, self.CFL, self.dt_fixed = Re, scheme, CFL, dt
-        self.sgs = sgs            # dict of rhs_sgs keyword arguments, or None
+        self.sgs = sgs            # dict of rhs_sgs keyword arguments (z0 for the wall model), or None
         self.forcing = forcing    # dict(kmin, kmax, nwaves, eps, seed), or None
-        self.spC = Spectral(nx, ny, nz, self.dx, self.dy, self.dz, periodic_z=True)
+        self.walls = walls        # dict(botWall, topWall, dpFdx, stokes_pressure), or None
+        self.per = walls is None
+        self.spC = Spectral(nx, ny, nz, self.dx, self.dy, self.dz, periodic_z=self.per)
         self.spE = Spectral(nx, ny, nz + 1, self.dx, self.dy, self.dz)
-        self.pp = PadePoissonPeriodic(self.spC, self.dx, self.dy, self.dz)
-        # slip-wall boundary kinds (get_boundary_conditions_stencil,
-        # igrid.F90:5148-5220, botWall = topWall = 2)
-        self.bc = (1, 1, 1, 1, -1, -1, -1, -1, -1, -1, 0, 0)
+        if self.per:
+            self.pp = PadePoissonPeriodic(self.spC, self.dx, self.dy, self.dz)
+            # slip-wall boundary kinds (get_boundary_conditions_stencil,
+            # igrid.F90:5148-5220, botWall = topWall = 2)
+            self.bc = (1, 1, 1, 1, -1, -1, -1, -1, -1, -1, 0, 0)
+        else:
+            self.pp = PadePoissonWalls(self.spC, self.dx, self.dy, self.dz,
+                                       Lz=Lz if walls.get("stokes_pressure") else None)
+            bc = [1, 1, 1, 1, -1, -1, -1, -1, -1, -1, 0, 0]
+            for side, wall in enumerate((walls["botWall"], walls["topWall"])):
+                # (uBC = vBC, wBC, dUdzBC = dVdzBC, dWdzBC) of each wall kind
+                uB, wB, dUB, dWB = {1: (-1, 1, 0, -1), 2: (1, -1, -1, 0), 3: (0, -1, 0, 0)}[wall]
+                bc[0 + side], bc[2 + side], bc[4 + side] = uB, uB, wB
+                bc[6 + side], bc[8 + side], bc[10 + side] = dUB, dUB, dWB
+            self.bc = tuple(bc)
         self.tsim, self.step_count, self.dt, self.draws = 0.0, 0, 0.0, 0
         self.newTimeStep = True
 
     # project_and_prep
     def prep(self, uh, vh, wh):
         sp = self.spC
-        uh, vh, wh = sp.dealias(uh), sp.dealias(vh), sp.dealias_edge(wh)
+        uh, vh = sp.dealias(uh), sp.dealias(vh)
+        wh = sp.dealias_edge(wh) if self.per else self.spE.dealias(wh)
         uh, vh, wh = self.pp.projection(uh, vh, wh)
         f = dict(uhat=uh, vhat=vh, what=wh, u=sp.ifft(uh), v=sp.ifft(vh), w=self.spE.ifft(wh))
-        f.update(interp_primitive(sp, self.spE, self.dz, True, self.bc, uh, vh, wh))
-        f.update(duidxj(sp, self.spE, self.dz, True, self.bc, self.Re <= 0, uh, vh, f["whatC"], wh, f["uEhat"],
+        f.update(interp_primitive(sp, self.spE, self.dz, self.per, self.bc, uh, vh, wh))
+        f.update(duidxj(sp, self.spE, self.dz, self.per, self.bc, self.Re <= 0, uh, vh, f["whatC"], wh, f["uEhat"],
                         f["vEhat"]))
         self.f = f
         return uh, vh, wh
@@ -113,7 +135,7 @@ class Igrid:
         """populate_rhs of the prepared stage."""
         f, spC, spE = self.f, self.spC, self.spE
         gC, gE = f["duidxjC"], f["duidxjE"]
-        E2C = lambda x: pade6_apply(STG_IN_E2C, x, self.dz, self.nz, 0, 0, True)  # noqa: E731
+        E2C = lambda x: pade6_apply(STG_IN_E2C, x, self.dz, self.nz, 0, 0, self.per)  # noqa: E731
         # AddNonLinearTerm_Rot
         ur = E2C(spE.fft((gE[..., 6] - gE[..., 2]) * f["w"])) + spC.fft((gC[..., 3] - gC[..., 1]) * f["v"])
         vr = E2C(spE.fft((gE[..., 7] - gE[..., 5]) * f["w"])) + spC.fft((gC[..., 1] - gC[..., 3]) * f["u"])
@@ -125,7 +147,14 @@ class Igrid:
             vr = vr + (1 / self.Re) * (-ks * f["vhat"] + f["d2vdz2hatC"])
             wr = wr + (1 / self.Re) * (-ks * f["what"] + f["d2wdz2hatE"])
         if self.sgs is not None:
-            ur, vr, wr = rhs_sgs(spC, spE, self.dz, True, gC, gE, ur, vr, wr, **self.sgs)[:3]
+            kw = dict(self.sgs)
+            z0 = kw.pop("z0", None)
+            if z0 is not None:
+                kw["wall"] = (f["u"], f["v"], f["uhat"], f["vhat"], z0)
+            ur, vr, wr = rhs_sgs(spC, spE, self.dz, self.per, gC, gE, ur, vr, wr, **kw)[:3]
+        if self.walls is not None and self.walls.get("dpFdx", 0.0) != 0.0:
+            ur = ur.copy()
+            ur[0, 0, :] += self.walls["dpFdx"] * self.nx * self.ny
         if self.forcing is not None:
             ur, vr, wr = self.forcing_rhs(ur, vr, wr)
         return ur, vr, wr

@@ -990,9 +990,10 @@ class cgrid:
 
 class igrid:
     """Time advancement of type(igrid) (src/incompressible/igrid.F90) on one
-    GPU for domains periodic in x, y and z: TVD-RK3 (scheme 1) or SSP-RK45
-    (scheme 2) stages of populate_rhs (rotational nonlinear term, viscous
-    term, optional SGS stress and HIT forcing) and project_and_prep."""
+    GPU for domains periodic in x, y and z, or with walls in z: TVD-RK3
+    (scheme 1) or SSP-RK45 (scheme 2) stages of populate_rhs (rotational
+    nonlinear term, viscous term, optional SGS stress with the wall model,
+    HIT forcing or a mean pressure gradient) and project_and_prep."""
 
     FIELDS = {"u": 0, "v": 1, "wC": 2, "w": 3, "uhat": 4, "vhat": 5, "what": 6, "duidxjC": 7, "duidxjE": 8,
               "urhs": 9, "vrhs": 10, "wrhs": 11, "uE": 12, "vE": 13, "whatC": 14}
@@ -1001,9 +1002,12 @@ class igrid:
         self._h = ctypes.c_void_p()
 
     def init(self, nx, ny, nz, Lx, Ly, Lz, Re=0.0, scheme=2, CFL=0.0, dt=0.0, sgs=None, forcing=None,
-             prec="f64"):
-        """sgs: dict(model, explicit_edge, camd, cglobal) or None; forcing:
-        dict(kmin, kmax, nwaves, eps, seed) or None."""
+             walls=None, prec="f64"):
+        """sgs: dict(model, explicit_edge, camd, cglobal, wall_model, z0) or
+        None; forcing: dict(kmin, kmax, nwaves, eps, seed) or None; walls:
+        dict(botWall, topWall, dpFdx, stokes_pressure) for walls in z (botWall
+        1 no-slip, 2 slip, 3 wall model with sgs wall_model "moeng"), or None
+        for periodic z."""
         p = _lib.IgridParams()
         p.nx, p.ny, p.nz, p.Lx, p.Ly, p.Lz = nx, ny, nz, Lx, Ly, Lz
         p.Re, p.scheme, p.CFL, p.dt = Re, scheme, CFL, dt
@@ -1013,10 +1017,17 @@ class igrid:
             p.sgs.explicit_edge = int(sgs.get("explicit_edge", False))
             p.sgs.camd_x, p.sgs.camd_y, p.sgs.camd_z = sgs.get("camd", (1.0, 1.0, 1.0))
             p.sgs.cglobal = sgs.get("cglobal", 1.0)
+            p.sgs.wall_model = {None: 0, "moeng": 1}[sgs.get("wall_model")]
+            p.sgs.z0 = sgs.get("z0", 0.0)
         if forcing is not None:
             p.use_forcing = 1
             p.kmin, p.kmax = forcing["kmin"], forcing["kmax"]
             p.nwaves, p.eps_amplitude, p.seed = forcing["nwaves"], forcing["eps"], forcing["seed"]
+        if walls is not None:
+            p.walls = 1
+            p.botWall, p.topWall = int(walls.get("botWall", 1)), int(walls.get("topWall", 1))
+            p.dpFdx = float(walls.get("dpFdx", 0.0))
+            p.stokes_pressure = int(bool(walls.get("stokes_pressure", False)))
         self.shape = (nx, ny, nz)
         self.prec = prec
         return _check(lib().pdo_igrid_init(ctypes.byref(self._h), ctypes.cast(ctypes.pointer(p), ctypes.c_void_p),

@@ -218,7 +218,9 @@ class IgridParams(ctypes.Structure):
                 ("Ly", ctypes.c_double), ("Lz", ctypes.c_double), ("Re", ctypes.c_double), ("scheme", ctypes.c_int),
                 ("CFL", ctypes.c_double), ("dt", ctypes.c_double), ("use_sgs", ctypes.c_int), ("sgs", SgsParams),
                 ("use_forcing", ctypes.c_int), ("kmin", ctypes.c_double), ("kmax", ctypes.c_double),
-                ("eps_amplitude", ctypes.c_double), ("nwaves", ctypes.c_int), ("seed", ctypes.c_ulonglong)]
+                ("eps_amplitude", ctypes.c_double), ("nwaves", ctypes.c_int), ("seed", ctypes.c_ulonglong),
+                ("walls", ctypes.c_int), ("botWall", ctypes.c_int), ("topWall", ctypes.c_int),
+                ("dpFdx", ctypes.c_double), ("stokes_pressure", ctypes.c_int)]
 
 
 class LibraryMissing(ImportError):

@@ -413,6 +413,25 @@ hipError_t launch_wall_model(int mode, const void *u, const void *v, void *tauE,
     return wall_t<float>(u, v, tauE, nxy, nE, part, nb, kappa, lg, wm, st);
 }
 
+namespace {
+template <typename TS>
+__global__ void __launch_bounds__(256) k_mean_add(TS *f, long nplane, int nz, double v)
+{
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < nz; k += gridDim.x * 256) {
+        TS *m = f + 2 * (long)k * nplane;
+        *m = (TS)((double)*m + v);
+    }
+}
+}  // namespace
+
+hipError_t launch_mean_add(int mode, void *f, long nplane, int nz, double v, hipStream_t st)
+{
+    const dim3 g((unsigned)((nz + 255) / 256));
+    if (mode == 0) hipLaunchKernelGGL((k_mean_add<double>), g, dim3(256), 0, st, (double *)f, nplane, nz, v);
+    else hipLaunchKernelGGL((k_mean_add<float>), g, dim3(256), 0, st, (float *)f, nplane, nz, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_clip_negative(int mode, void *a, long n, hipStream_t st)
 {
     if (mode == 0)

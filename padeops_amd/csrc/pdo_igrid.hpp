@@ -14,6 +14,9 @@ hipError_t launch_surface_plane(int mode, void *f, long nplane, double v, hipStr
 // tau = -2 nu S: cells (nC points) 11, 12, 22, 33; edges (nE points) 13, 23
 hipError_t launch_tau(int mode, const void *nuC, const void *SC, const void *nuE, const void *SE, void *tauC,
                       void *tauE, long nC, long nE, hipStream_t st);
+// f[k nplane] += v (the real part of the mean mode of each of nz planes of
+// a spectrum with nplane complex values per plane)
+hipError_t launch_mean_add(int mode, void *f, long nplane, int nz, double v, hipStream_t st);
 // a = max(a, 0) (real)
 hipError_t launch_clip_negative(int mode, void *a, long n, hipStream_t st);
 // rhs -= i k f (kv per index along dim 1 or 2) or rhs -= f (kv NULL); complex

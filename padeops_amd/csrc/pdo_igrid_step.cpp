@@ -1,5 +1,6 @@
 // Time advancement of type(igrid) on one GPU, for domains periodic in z (the
-// forced isotropic turbulence setup): src/incompressible/igrid.F90
+// forced isotropic turbulence setup) or with walls in z (the channel and
+// boundary-layer setups): src/incompressible/igrid.F90
 //   timeAdvance :1056-1076 -> TVD_RK3 :1105-1170 / SSP_RK45 :1172-1299
 //   populate_rhs :1793-1912: AddNonLinearTerm_Rot :1497-1566, addViscousTerm
 //     :1914-1959, sgs_igrid%getRHS_SGS, HIT_shell_forcing%getRHS_HITforcing
@@ -110,11 +111,12 @@ namespace {
 int project_and_prep(pdo_igrid h, void *uh, void *vh, void *wh, hipStream_t st)
 {
     void *s = (void *)st;
-    // Step 1: dealiasFields (periodic in z: the 3D mask on cells, the edge
-    // variant on w)
+    // Step 1: dealiasFields (:1020-1037; periodic in z: the 3D mask on cells,
+    // the edge variant on w; with walls the x-y mask of each object)
     PDO_TRY(pdo_spectral_dealias(h->spC, h->prec, uh, s));
     PDO_TRY(pdo_spectral_dealias(h->spC, h->prec, vh, s));
-    PDO_TRY(pdo_spectral_dealias_edge(h->spC, h->prec, wh, s));
+    if (h->p.walls) PDO_TRY(pdo_spectral_dealias(h->spE, h->prec, wh, s));
+    else PDO_TRY(pdo_spectral_dealias_edge(h->spC, h->prec, wh, s));
     // Step 2: pressure projection
     PDO_TRY(pdo_padepoisson_projection(h->pp, h->prec, uh, vh, wh, s));
     // Step 3: physical fields
@@ -189,8 +191,15 @@ int populate_rhs(pdo_igrid h, void *ur, void *vr, void *wr, hipStream_t st)
                                         h->dlam, s));
             lam = h->dlam;
         }
-        PDO_TRY(pdo_sgsrhs_compute(h->sgs, h->prec, h->gC, h->gE, lam, ur, vr, wr, s));
+        if (h->p.sgs.wall_model)
+            PDO_TRY(pdo_sgsrhs_compute_wall(h->sgs, h->prec, h->gC, h->gE, lam, h->u, h->v, ur, vr, wr, s));
+        else
+            PDO_TRY(pdo_sgsrhs_compute(h->sgs, h->prec, h->gC, h->gE, lam, ur, vr, wr, s));
     }
+    // Step 3a: addExtraForcingTerm (:1759-1765): the transform of the
+    // uniform dpFdx, i.e. dpFdx nx ny on the mean mode of every plane of u
+    if (h->p.dpFdx != 0.0)
+        PDO_HIP(launch_mean_add(mode, ur, (long)h->nxh * h->ny, h->nz, h->p.dpFdx * h->nx * h->ny, st), "forcing");
     // Step 8: HIT forcing; new wave vectors at the first stage of a step
     if (h->p.use_forcing) {
         if (h->newTimeStep) h->draws++;
@@ -312,6 +321,17 @@ int pdo_igrid_init(pdo_igrid *out, const pdo_igrid_params *p, int prec)
     if (mode < 0) { set_error("unknown precision mode"); return PDO_ERR_ARG; }
     if (p->scheme != 1 && p->scheme != 2) { set_error("timeSteppingScheme: 1 TVD-RK3 or 2 SSP-RK45"); return PDO_ERR_ARG; }
     if (!(p->CFL > 0) && !(p->dt > 0)) { set_error("a CFL number or a fixed time step is needed"); return PDO_ERR_ARG; }
+    if (p->walls) {
+        if (p->botWall < 1 || p->botWall > 3 || p->topWall < 1 || p->topWall > 2) {
+            set_error("walls: botWall 1 no-slip, 2 slip or 3 wall model; topWall 1 or 2");
+            return PDO_ERR_ARG;
+        }
+        if ((p->botWall == 3) != (p->use_sgs && p->sgs.wall_model == 1)) {
+            set_error("walls: a wall-model bottom wall (botWall 3) goes with the SGS wall model, and only then");
+            return PDO_ERR_ARG;
+        }
+        if (p->use_forcing) { set_error("walls: HIT forcing needs periodic z"); return PDO_ERR_ARG; }
+    }
     if (p->use_forcing) {
         const int kc = (int)std::ceil(p->kmax);
         if (p->nwaves < 1 || !(p->kmin >= 0) || !(p->kmax >= p->kmin) || kc >= p->nx / 2 || kc >= p->ny ||
@@ -329,17 +349,36 @@ int pdo_igrid_init(pdo_igrid *out, const pdo_igrid_params *p, int prec)
     h->dx = p->Lx / p->nx; h->dy = p->Ly / p->ny; h->dz = p->Lz / p->nz;
     int st = 0;
     // igrid.F90:489-493 (periodic in z: the cell object carries the 3D dealias
-    // mask), Pade6stagg periodic, the periodic PadePoisson
-    if (!st) st = pdo_spectral_init(&h->spC, h->nx, h->ny, h->nz, h->dx, h->dy, h->dz, 0, 2.0 / 3.0, 1);
+    // mask), Pade6stagg and PadePoisson periodic or with walls
+    const int per = p->walls ? 0 : 1;
+    if (!st) st = pdo_spectral_init(&h->spC, h->nx, h->ny, h->nz, h->dx, h->dy, h->dz, 0, 2.0 / 3.0, per);
     if (!st) st = pdo_spectral_init(&h->spE, h->nx, h->ny, h->nz + 1, h->dx, h->dy, h->dz, 0, 2.0 / 3.0, 0);
-    if (!st) st = pdo_pade6stagg_init(&h->d6, h->nz, h->dz, 1);
-    if (!st) st = pdo_padepoisson_init_periodic(&h->pp, h->spC, h->dz);
+    if (!st) st = pdo_pade6stagg_init(&h->d6, h->nz, h->dz, per);
     if (!st) {
-        // boundary kinds of slip walls (botWall = topWall = 2,
-        // get_boundary_conditions_stencil :5148-5220), the setting of the
-        // periodic isotropic runs; with periodic z they only select
-        // d2dz2_C2C for the second z derivatives of u and v
-        const int bc[12] = {1, 1, 1, 1, -1, -1, -1, -1, -1, -1, 0, 0};
+        if (per) {
+            st = pdo_padepoisson_init_periodic(&h->pp, h->spC, h->dz);
+        } else {
+            st = pdo_padepoisson_init_nonperiodic(&h->pp, h->spC, h->dz);
+            if (!st && p->stokes_pressure) st = pdo_padepoisson_set_stokes(h->pp, p->Lz);
+        }
+    }
+    if (!st) {
+        // boundary kinds of get_boundary_conditions_stencil (:5148-5220):
+        // {uBC, vBC, wBC, dUdzBC, dVdzBC, dWdzBC} x {bottom, top}; slip walls
+        // for periodic z (the setting of the periodic isotropic runs, where
+        // they only select d2dz2_C2C for the second z derivatives of u, v)
+        int bc[12] = {1, 1, 1, 1, -1, -1, -1, -1, -1, -1, 0, 0};
+        if (p->walls) {
+            for (int side = 0; side < 2; side++) {
+                const int wall = side == 0 ? p->botWall : p->topWall;
+                int uB, wB, dUB, dWB;
+                if (wall == 1) { uB = -1; wB = 1; dUB = 0; dWB = -1; }      // no-slip
+                else if (wall == 2) { uB = 1; wB = -1; dUB = -1; dWB = 0; }  // slip
+                else { uB = 0; wB = -1; dUB = 0; dWB = 0; }                  // wall model
+                bc[0 + side] = uB; bc[2 + side] = uB; bc[4 + side] = wB;
+                bc[6 + side] = dUB; bc[8 + side] = dUB; bc[10 + side] = dWB;
+            }
+        }
         st = pdo_duidxj_init(&h->dj, h->spC, h->spE, h->d6, bc, p->Re > 0 ? 0 : 1);
     }
     if (!st && p->use_sgs) st = pdo_sgsrhs_init(&h->sgs, h->spC, h->spE, h->d6, &p->sgs, prec);

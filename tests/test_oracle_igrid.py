@@ -102,3 +102,41 @@ def test_forcing_power_input():
     nz_modes = np.argwhere(mag > 1e-10 * mag.max())
     assert 1 <= len(nz_modes) <= F["nwaves"]
     assert (nz_modes <= np.ceil(F["kmax"])).all()
+
+
+def test_walls_uniform_acceleration_between_slip_walls():
+    """walls in z, slip top and bottom, fluid at rest, mean pressure gradient
+    dpFdx: a uniform u is even about both walls, so its viscous term is zero
+    and u = dpFdx t exactly (v = w = 0) under both Runge-Kutta schemes."""
+    nx, ny, nz, dp = 8, 8, 16, 0.7
+    for scheme in (1, 2):
+        g = Igrid(nx, ny, nz, L, L, 1.0, Re=50.0, scheme=scheme, dt=0.01,
+                  walls=dict(botWall=2, topWall=2, dpFdx=dp))
+        z = np.zeros((nx, ny, nz))
+        g.set_velocity(z, z, np.zeros((nx, ny, nz + 1)))
+        for _ in range(3):
+            g.step()
+        f = g.f
+        assert np.abs(f["u"] - dp * g.tsim).max() <= 1e-13
+        assert np.abs(f["v"]).max() <= 1e-14 and np.abs(f["w"]).max() <= 1e-14
+
+
+def test_walls_poiseuille_profile_is_steady_away_from_the_walls():
+    """no-slip walls: the Poiseuille parabola u = (Re dpFdx / 2) z (Lz - z)
+    balances the pressure gradient wherever the compact second derivative is
+    exact (a quadratic), i.e. away from the walls, where the odd reflection of
+    the no-slip kind acts; one step leaves the centre cells unchanged.  A
+    wrong sign or scale of the forcing or of the viscous term moves them."""
+    nx, ny, nz, Re, dp, Lz = 8, 8, 32, 20.0, 1.0, 1.0
+    g = Igrid(nx, ny, nz, L, L, Lz, Re=Re, scheme=2, dt=0.002, walls=dict(botWall=1, topWall=1, dpFdx=dp))
+    zc = (np.arange(nz) + 0.5) * Lz / nz
+    prof = 0.5 * Re * dp * zc * (Lz - zc)
+    u0 = np.broadcast_to(prof[None, None, :], (nx, ny, nz)).copy()
+    z = np.zeros((nx, ny, nz))
+    g.set_velocity(u0, z, np.zeros((nx, ny, nz + 1)))
+    g.step()
+    du = g.f["u"] - u0
+    centre = slice(nz // 2 - 3, nz // 2 + 3)
+    assert np.abs(du[:, :, centre]).max() <= 1e-9 * prof.max()
+    assert np.abs(du).max() > 1e-6 * prof.max()          # the walls do act
+    assert np.abs(g.f["v"]).max() <= 1e-13 and np.abs(g.f["w"]).max() <= 1e-12 * prof.max()
