@@ -48,6 +48,8 @@ def main():
                     help="only launch this pass type (cd06_x .. cf90_z, poisson_5pass) --launches times, then a "
                          "1 GiB device copy as many times (the byte-counter calibration of rocprofv3 --pmc runs)")
     ap.add_argument("--launches", type=int, default=5)
+    ap.add_argument("--only", choices=["wall_les"], default=None,
+                    help="run only this secondary workload and print its JSON (the wall-modelled LES step)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -63,6 +65,10 @@ def main():
     n = args.n
     if args.profile_pass:
         profile_pass(P, n, dev, args.profile_pass, args.launches)
+        return
+    if args.only == "wall_les":
+        print(json.dumps({"wall_les_512x512x256": wall_les_bench(P, dev, torch.cuda.current_stream().cuda_stream)}),
+              flush=True)
         return
     N = n ** 3
     L = 2 * np.pi
@@ -146,6 +152,10 @@ def main():
     # the compressible (cgrid) right-hand side at 256^3
     comp = None if args.no_breakdown else cgrid_bench(P, dev, stream, max(3, args.steps // 4))
     torch.cuda.empty_cache()
+    # the wall-modelled LES step of the 512 x 512 x 256 box with the learned
+    # closure in the loop
+    les = None if args.no_breakdown else wall_les_bench(P, dev, stream)
+    torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
@@ -182,6 +192,7 @@ def main():
         "pass_ms": breakdown,
         "wall_box_512x512x256": wall,
         "cgrid_rhs_256": comp,
+        "wall_les_512x512x256": les,
         "hbm_probe": probe,
         "cpu_baseline": cpu,
         "wall_s_timed_fp32r": round(wall32, 4),
@@ -272,6 +283,79 @@ def cgrid_bench(P, dev, stream, steps):
     out["fp32r_rel_l2_vs_fp64"] = {c: float("%.3e" % rel_l2(res["f32_refine"][c * N:(c + 1) * N],
                                                             res["f64"][c * N:(c + 1) * N]))
                                    for c in range(5)}
+    return out
+
+
+def wall_les_bench(P, dev, stream, steps=3):
+    """One SSP-RK45 step of the wall-modelled LES on the 512 x 512 x 256 box
+    (periodic x, y of side 2 pi; walls in z, dz = 2 pi / 512: the Moeng wall
+    model below, a slip wall above, dpFdx = 1; log-law mean profile with
+    z0 = 1e-4 plus the seeded turbulence; FP32+refine), with the learned
+    closure at every stage and the dynamic Smagorinsky model where its
+    dissipation is negative, using the network trained on the solver's own
+    256^3 DNS (results/closure_c4_dns256_params.npy).  Reports ms per step
+    (five stages, each a projection, the right-hand side, the dynamic
+    procedure and the closure on 67 M cell and edge points), the closure's
+    fallback share at the last stage and whether the run stayed finite."""
+    import math
+    import os
+
+    import numpy as np
+    import torch
+    from padeops_amd.synthetic import wall_bounded
+    nx, nz = 512, 256
+    dz = 2 * math.pi / nx
+    Lz = nz * dz
+    g = P.igrid()
+    g.init(nx, nx, nz, 2 * math.pi, 2 * math.pi, Lz, Re=0.0, scheme=2, CFL=0.5,
+           sgs=dict(model="smagorinsky", cglobal=0.01, wall_model="moeng", z0=1e-4),
+           walls=dict(botWall=3, topWall=2, dpFdx=1.0), prec="f32_refine")
+    uC, _, wE = wall_bounded(nx, nx, nz, 8.0, 1234, torch.float32, dev)
+    wE.view(nz + 1, -1)[0].zero_()
+    wE.view(nz + 1, -1)[nz].zero_()
+    g.set_velocity(uC, torch.zeros_like(uC), wE)
+    del uC, wE
+    params = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "results",
+                                  "closure_c4_dns256_params.npy")).astype(np.float32)
+    net = P.SGSNet(64)
+    net.set_params(params)
+    npts = nx * nx * (nz + 1)
+    cl = P.Closure(net, npts)
+    delta = (2 * math.pi / nx * 2 * math.pi / nx * dz) ** (1.0 / 3.0)
+    g.set_closure(2, cl, delta=delta, dratio=delta / (math.pi / 21))
+    g.step(1)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    g.step(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    u = g.field("u")
+    finite = bool(torch.isfinite(u).all().item())
+    out = {"ms_per_step": round(ms, 2), "steps_per_s": round(1000.0 / ms, 3), "stages_per_step": 5,
+           "prec": "f32_refine", "closure": "learned + dynamic fallback", "finite": finite,
+           "fallback_share": round(cl.fallback_count() / npts, 4), "t": round(g.time()[0], 5)}
+    del g, cl
+    torch.cuda.empty_cache()
+    # the network alone on the stage's 67 M points (k_mlp_infer): matrix-core
+    # work of the padded layers 16-64-64-64-16, two flops per multiply-add
+    X = torch.randn((npts, 10), dtype=torch.float32, device=dev)
+    Y = torch.empty((npts, 6), dtype=torch.float32, device=dev)
+    net.forward(X, Y)
+    reps = 10
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        net.forward(X, Y)
+    e1.record()
+    torch.cuda.synchronize()
+    mlp_ms = e0.elapsed_time(e1) / reps
+    flop = 2.0 * npts * (16 * 64 + 2 * 64 * 64 + 64 * 16)
+    out["mlp_infer"] = {"points": npts, "ms": round(mlp_ms, 3), "tflops": round(flop / mlp_ms / 1e9, 1),
+                        "frac_bf16_peak": round(flop / mlp_ms / 1e9 / 2500.0, 4),
+                        "gelu_evals_T_per_s": round(192.0 * npts / mlp_ms / 1e9, 3)}
+    del net, X, Y
     return out
 
 

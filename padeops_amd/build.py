@@ -34,9 +34,14 @@ def flags():
             "-I", os.path.join(HERE, "..", "include"), "-I", CSRC]
 
 
+# per-file extras: the MLP kernels take their MFMA results in VGPRs (no
+# accumulator-register copies on the way to the GELU epilogue)
+FILE_FLAGS = {"pdo_mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [hipcc()] + flags() + ["-c", src, "-o", obj]
+    cmd = [hipcc()] + flags() + FILE_FLAGS.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))

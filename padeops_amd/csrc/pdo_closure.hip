@@ -16,7 +16,8 @@
 //     T5 = r s^2 - s^2 r, T6 = r^2 s + s r^2 - 2/3 tr(s r^2) I;
 //   stress tau = Delta^2 g^2 sum_n G_n T_n, G the network's 6 outputs.
 // The stress is frame-indifferent: rotating A rotates tau and leaves x and G
-// unchanged.  Arithmetic is FP64 per point (full rate on gfx950).
+// unchanged.  Arithmetic is FP64 per point (full rate on gfx950), except the
+// FP32 roots and principal-strain trigonometry of the features.
 #include <hip/hip_runtime.h>
 
 #include "pdo_closure.hpp"
@@ -70,34 +71,35 @@ __device__ __forceinline__ void strain_rotation(const TS *g, long t, long stride
         }
 }
 
-__device__ __forceinline__ double sroot(double v, double p) { return v >= 0 ? pow(v, 1.0 / p) : -pow(-v, 1.0 / p); }
-
+// The invariants (traces, where cancellation lives) are FP64; their roots and
+// the principal-strain trigonometry are FP32, whose ~1e-7 relative error is
+// far below the BF16 rounding the network applies to its inputs.
 __device__ void features_of(const M3 &s, const M3 &r, double dratio, float *x)
 {
     const M3 s2 = mul(s, s), r2 = mul(r, r);
     const double i1 = tr(s2), i2 = -tr(r2), i3 = tr(mul(s2, s)), i4 = tr(mul(r2, s)), i5 = tr(mul(r2, s2));
     const double i6 = tr(mul(mul(r2, s), mul(r, s2)));
-    x[0] = (float)sqrt(i1 > 0 ? i1 : 0.0);
-    x[1] = (float)sqrt(i2 > 0 ? i2 : 0.0);
-    x[2] = (float)cbrt(i3);
-    x[3] = (float)cbrt(i4);
-    x[4] = (float)sroot(-i5 > 0 ? -i5 : 0.0, 4.0);
-    x[5] = (float)sroot(i6, 6.0);
+    x[0] = sqrtf(i1 > 0 ? (float)i1 : 0.0f);
+    x[1] = sqrtf(i2 > 0 ? (float)i2 : 0.0f);
+    x[2] = cbrtf((float)i3);
+    x[3] = cbrtf((float)i4);
+    x[4] = sqrtf(sqrtf(-i5 > 0 ? (float)-i5 : 0.0f));
+    x[5] = copysignf(cbrtf(sqrtf(fabsf((float)i6))), (float)i6);
     // principal strains: roots of l^3 - p l - q = 0, p = tr(s^2)/2, q = det s = tr(s^3)/3
     const double p = 0.5 * i1, q = i3 / 3.0;
-    double l0 = 0, l1 = 0, l2 = 0;
+    float l0 = 0, l1 = 0, l2 = 0;
     if (p > 1e-300) {
         double c = 1.5 * q / p * sqrt(3.0 / p);
         c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
-        const double ph = acos(c) / 3.0, m = 2.0 * sqrt(p / 3.0);
-        l0 = m * cos(ph);
-        l1 = m * cos(ph - 2.0943951023931955);   // 2 pi / 3
-        l2 = m * cos(ph - 4.1887902047863905);   // 4 pi / 3
+        const float ph = acosf((float)c) / 3.0f, m = 2.0f * sqrtf((float)(p / 3.0));
+        l0 = m * cosf(ph);
+        l1 = m * cosf(ph - 2.0943951023931955f);   // 2 pi / 3
+        l2 = m * cosf(ph - 4.1887902047863905f);   // 4 pi / 3
     }
     // descending order: l0 >= l1 >= l2 for ph in [0, pi/3]
-    x[6] = (float)l0;
-    x[7] = (float)l1;
-    x[8] = (float)l2;
+    x[6] = l0;
+    x[7] = l1;
+    x[8] = l2;
     x[9] = (float)dratio;
 }
 

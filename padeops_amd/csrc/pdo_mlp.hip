@@ -3,8 +3,8 @@
 // BF16 weights and activations, FP32 accumulation on the matrix cores
 // (v_mfma_f32_16x16x16_bf16), FP32 master parameters updated by Adam.
 //
-// A workgroup (4 waves) owns a tile of 64 samples and runs the whole network
-// on it with every weight matrix staged in LDS; wave w owns rows 16w..16w+15
+// Training: a workgroup (4 waves) owns a tile of 64 samples and runs the whole
+// network on it with every weight matrix staged in LDS; wave w owns rows 16w..16w+15
 // of the activation tile.  The backward kernel forms per-workgroup weight
 // gradients (the batch is the contraction dimension of dW = dZ^T H) that a
 // second kernel sums in a fixed order, so training is deterministic.
@@ -133,6 +133,141 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float *__restrict__ X
 }
 
 // ---------------------------------------------------------------------------
+// inference forward (no cache), the closure's evaluation on whole fields.
+// Each wave runs 16 samples at a time through the network in the transposed
+// form Z^T = W H^T: the weights are the MFMA A operand and the activations
+// the B operand, so lane l ends a 16-row output tile holding
+// Z(16 nt + 4 (l / 16) + i, sample l % 16), i < 4 -- exactly the B operand
+// of the next layer's k step nt.  Activations therefore never leave the
+// registers.  The BF16 weights of all four layers are staged in LDS once per
+// (persistent) workgroup in fragment order, lane l's 8 bytes of a fragment
+// holding W(16 nt + l % 16, 16 ks + 4 (l / 16) + i), i < 4, so each A
+// operand is one conflict-free ds_read_b64 and the kernel needs ~66 VGPRs
+// (held in registers instead, the weights take 80 VGPRs and cut the waves per
+// SIMD from 7 to 2, which measured 17% slower).  The biases, staged in LDS
+// too, seed the accumulators.  BF16 rounding is the hardware's
+// round-to-nearest-even conversion (the values of to_bf16 for finite
+// inputs).  GELU is evaluated as z sigma(2 y) = z / (1 + 2^(-2 y log2 e)),
+// y = sqrt(2/pi) (z + 0.044715 z^3), identical to 0.5 z (1 + tanh y); with
+// the hardware exp2 and reciprocal it is within a few FP32 ulp of the tanhf
+// form, far below the BF16 rounding of the activation.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16v4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x4 pack_bf16(f32x4 v)
+{
+    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bf16v4));
+}
+__device__ __forceinline__ f32x4 unpack_bf16(bf16x4 v)
+{
+    f32x4 f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) f[i] = from_bf16((unsigned short)v[i]);
+    return f;
+}
+
+__device__ __forceinline__ float gelu_fast(float z)
+{
+    constexpr float c1 = (float)(-2.0 * 0.7978845608028654 * 1.4426950408889634);
+    constexpr float c3 = (float)(-2.0 * 0.7978845608028654 * 1.4426950408889634 * 0.044715);
+    const float a = z * __builtin_fmaf(z * z, c3, c1);
+    return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(a));
+}
+
+// bias + BF16 pre-activation + GELU + BF16 activation of one output tile
+__device__ __forceinline__ bf16x4 hidden_act(f32x4 z)
+{
+    const f32x4 zr = unpack_bf16(pack_bf16(z));
+    f32x4 h;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) h[i] = gelu_fast(zr[i]);
+    return pack_bf16(h);
+}
+
+constexpr int INFER_WAVES = 4;  // waves per workgroup
+constexpr int INFER_NFRAG = 4 + 16 + 16 + 4;  // A-operand fragments of the four layers
+
+// fragment f of the A operands: layer 1 nt (f < 4), layer 2 (nt, ks) (4 + 4 nt + ks),
+// layer 3 (20 + 4 nt + ks), layer 4 ks (36 + ks); lane l holds
+// W(16 nt + l % 16, 16 ks + 4 (l / 16) + i), i < 4
+__device__ __forceinline__ bf16x4 weight_frag(const unsigned short *Wb, int f, int lane)
+{
+    int off, K, nt, ks;
+    if (f < 4) off = MLP_OFF_W1, K = MLP_IN_PAD, nt = f, ks = 0;
+    else if (f < 20) off = MLP_OFF_W2, K = MLP_HID, nt = (f - 4) >> 2, ks = (f - 4) & 3;
+    else if (f < 36) off = MLP_OFF_W3, K = MLP_HID, nt = (f - 20) >> 2, ks = (f - 20) & 3;
+    else off = MLP_OFF_W4, K = MLP_HID, nt = 0, ks = f - 36;
+    const unsigned short *q = Wb + off + (16 * nt + (lane & 15)) * K + 16 * ks + 4 * (lane >> 4);
+    bf16x4 v;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = (short)q[i];
+    return v;
+}
+
+__global__ void __launch_bounds__(64 * INFER_WAVES)
+k_mlp_infer(const float *__restrict__ X, float *__restrict__ Y, const unsigned short *__restrict__ Wb,
+            const float *__restrict__ P, long B, long ntiles)
+{
+    constexpr int NB = 3 * MLP_HID + MLP_OUT_PAD;
+    __shared__ __align__(16) float sB[NB];
+    __shared__ __align__(16) bf16x4 sW[INFER_NFRAG * 64];
+    for (int i = threadIdx.x; i < NB; i += blockDim.x) {
+        const int l = i / MLP_HID;
+        const int off = l == 0 ? MLP_OFF_B1 : l == 1 ? MLP_OFF_B2 : l == 2 ? MLP_OFF_B3 : MLP_OFF_B4;
+        sB[i] = P[off + i - l * MLP_HID];
+    }
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    for (int i = threadIdx.x; i < INFER_NFRAG * 64; i += blockDim.x) sW[i] = weight_frag(Wb, i >> 6, i & 63);
+    __syncthreads();
+    auto W = [&](int f) { return sW[f * 64 + lane]; };
+    auto bias = [&](int l, int nt) { return *(const f32x4 *)(sB + l * MLP_HID + 16 * nt + 4 * g); };
+    const long stride = (long)gridDim.x * INFER_WAVES;
+    for (long t = (long)blockIdx.x * INFER_WAVES + (threadIdx.x >> 6); t < ntiles; t += stride) {
+        // keeps the LDS weight reads inside the loop (hoisted, they would
+        // occupy the registers the LDS placement is meant to free)
+        asm volatile("" ::: "memory");
+        const long s = 16 * t + (lane & 15);  // this lane's sample
+        // B operand of layer 1: X(s, 4 g + i), zero beyond the 10 inputs and the batch
+        f32x4 xv = {0, 0, 0, 0};
+        if (s < B) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (4 * g + i < MLP_IN) xv[i] = X[s * MLP_IN + 4 * g + i];
+        }
+        const bf16x4 x = pack_bf16(xv);
+        bf16x4 h[4], h2[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+            h[nt] = hidden_act(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W(nt), x, bias(0, nt), 0, 0, 0));
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            f32x4 acc = bias(1, nt);
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W(4 + 4 * nt + ks), h[ks], acc, 0, 0, 0);
+            h2[nt] = hidden_act(acc);
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            f32x4 acc = bias(2, nt);
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W(20 + 4 * nt + ks), h2[ks], acc, 0, 0, 0);
+            h[nt] = hidden_act(acc);
+        }
+        f32x4 y = bias(3, 0);
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W(36 + ks), h[ks], y, 0, 0, 0);
+        // lane (r, g) holds outputs 4 g .. 4 g + 3 of sample s; 6 are used
+        if (s < B && g < 2) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (4 * g + i < MLP_OUT) Y[s * MLP_OUT + 4 * g + i] = y[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // backward: per-workgroup gradient partials of the mean squared error
 // loss = mean over B x 6 of (Y - T)^2, or of any loss whose gradient with
 // respect to the outputs is dY; partial[blockIdx.x][MLP_NPARAM]
@@ -252,6 +387,24 @@ hipError_t launch_mlp_forward(const float *X, float *Y, const unsigned short *Wb
                               const MlpCache &cache, hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
+    if (!cache.H[0]) {
+        // inference: as many persistent workgroups as are resident at once
+        static int resident = 0;
+        if (!resident) {
+            int dev = 0, ncu = 0, per_cu = 0;
+            hipError_t e = hipGetDevice(&dev);
+            if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+            if (e == hipSuccess)
+                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mlp_infer, 64 * INFER_WAVES, 0);
+            if (e != hipSuccess) return e;
+            resident = ncu * (per_cu > 0 ? per_cu : 1);
+        }
+        const long ntiles = (B + 15) / 16;  // 16 samples per wave
+        const long need = (ntiles + INFER_WAVES - 1) / INFER_WAVES;
+        const long nwg = need < resident ? need : resident;
+        hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)nwg), dim3(64 * INFER_WAVES), 0, st, X, Y, Wb, P, B, ntiles);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_mlp_forward, dim3(nblocks(B)), dim3(256), 0, st, X, Y, Wb, P, B, cache);
     return hipGetLastError();
 }

@@ -2,7 +2,7 @@
 the network's backward pass from a given dL/dY (pdo_mlp_backward) against the
 oracle (oracle/closure.py, oracle/mlp.py).
 
-  features: FP32 outputs of FP64 arithmetic; the invariants (roots undone)
+  features: FP64 invariants, FP32 roots; the invariants (roots undone)
             to <= 2e-6 of each column's scale
   stress:   FP64 fields to 1e-12 relative; the igrid cell (11 12 22 33) and
             edge (13 23) layouts; the fallback keeps the prior values at
