@@ -1,11 +1,13 @@
 ! ISO_C_BINDING shim: the reference's Fortran derivative / filter / Poisson
 ! types over the C ABI of include/padeops_hip.h.  A solver switches with
 !     use padeops_hip_mod, only: cd06, cd10, cf90, cd06stagg, poissonperiodic, &
-!                                derivatives, filters, spectral, Pade6stagg, padepoisson
+!                                derivatives, filters, spectral, Pade6stagg, padepoisson, &
+!                                cgrid, cgrid_params
 ! in place of the modules of src/derivatives/cd06.F90, cd10.F90,
 ! cd06stagg.F90, derivatives.F90, src/filters/cf90.F90, filters.F90,
 ! src/utilities/PoissonPeriodic.F90 and src/incompressible/spectral.F90,
-! PadeDerOps.F90, PadePoisson.F90, and uses padeops_hip_decomp for the
+! PadeDerOps.F90, PadePoisson.F90 (and cgrid's right-hand side and filter,
+! src/compressible/cgrid.F90), and uses padeops_hip_decomp for the
 ! decomp_info / transpose_* of 2DECOMP on one GPU.
 ! Field arguments are device arrays (allocated with pdo_alloc below and
 ! viewed through c_f_pointer); the shim passes their addresses, it never
@@ -205,6 +207,26 @@ module padeops_hip_mod
             integer(c_int), value :: prec
         end function
         integer(c_int) function c_poisson_destroy(h) bind(C, name="pdo_poisson_destroy")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function c_cg_init(h, params, prec) bind(C, name="pdo_cgrid_init")
+            import :: c_ptr, c_int
+            type(c_ptr) :: h
+            type(c_ptr), value :: params
+            integer(c_int), value :: prec
+        end function
+        integer(c_int) function c_cg_rhs(h, prec, W, rhs, stream) bind(C, name="pdo_cgrid_rhs")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: h, W, rhs, stream
+            integer(c_int), value :: prec
+        end function
+        integer(c_int) function c_cg_filter(h, prec, arr, numtimes, gaussian, stream) bind(C, name="pdo_cgrid_filter")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: h, arr, stream
+            integer(c_int), value :: prec, numtimes, gaussian
+        end function
+        integer(c_int) function c_cg_destroy(h) bind(C, name="pdo_cgrid_destroy")
             import :: c_ptr, c_int
             type(c_ptr), value :: h
         end function
@@ -430,6 +452,42 @@ module padeops_hip_mod
         procedure :: getPressureAndUpdateRHS => pp_get_pressure_update
         procedure :: DivergenceCheck => pp_divergence
         procedure :: destroy => pp_destroy
+    end type
+
+    ! The settings of cgrid%init (cgrid.F90:113-760) that its right-hand side
+    ! reads, one species of ideal gas, laid out as pdo_cgrid_params: the
+    ! caller fills it from the values its input file gave (derivative 1 = cd10,
+    ! 2 = cd06; filter 1 = cf90; periodic 0/1; x_bc, y_bc, z_bc the (bc1, bcn)
+    ! of each direction; inviscid 1 = no physical viscosity, else the power law
+    ! mu_ref (T/T_ref)^visc_n with Prandtl number Pr).
+    type, bind(C), public :: cgrid_params
+        integer(c_int) :: nx = 0, ny = 0, nz = 0
+        real(c_double) :: dx = 0, dy = 0, dz = 0
+        integer(c_int) :: periodic(3) = 1
+        integer(c_int) :: x_bc(2) = 0, y_bc(2) = 0, z_bc(2) = 0
+        integer(c_int) :: derivative(3) = 1
+        integer(c_int) :: filter(3) = 1
+        real(c_double) :: gam = 1.4d0, Rgas = 1.d0
+        real(c_double) :: Cmu = 0.002d0, Cbeta = 1.75d0, Ckap = 0.01d0
+        integer(c_int) :: inviscid = 1
+        real(c_double) :: mu_ref = 1.d0, T_ref = 1.d0, visc_n = 0.75d0, Pr = 0.75d0
+    end type
+
+    ! cgrid (cgrid.F90:26-111): the right-hand side of the conserved variables
+    ! (getRHS :822-894 with get_primitive, getRHS_x/y/z, getLAD, get_tau and
+    ! get_q) and filter (:1259-1336).  The reference's getRHS is private to its
+    ! advance_RK45; a solver calls this one there, on device arrays
+    ! W, rhs (nxp, nyp, nzp, 5) = rho, rho u, rho v, rho w, TE.
+    type, public :: cgrid
+        type(c_ptr) :: h = c_null_ptr
+        integer :: nxp = 0, nyp = 0, nzp = 0
+        integer :: prec = PDO_F64
+        type(c_ptr) :: stream = c_null_ptr
+    contains
+        procedure :: init => cg_init
+        procedure :: getRHS => cg_get_rhs
+        procedure :: filter => cg_filter
+        procedure :: destroy => cg_destroy
     end type
 
     interface
@@ -1794,6 +1852,54 @@ contains
     subroutine pp_destroy(this)
         class(padepoisson), intent(inout) :: this
         if (c_associated(this%h)) call check(c_pp_destroy(this%h))
+        this%h = c_null_ptr
+    end subroutine
+
+    ! ---------------------------------------------------------------- cgrid
+    function cg_init(this, params, prec) result(ierr)
+        class(cgrid), intent(inout) :: this
+        type(cgrid_params), intent(in), target :: params
+        integer, optional, intent(in) :: prec
+        integer :: ierr
+        if (present(prec)) this%prec = prec
+        if (c_associated(this%h)) call check(c_cg_destroy(this%h))
+        this%h = c_null_ptr
+        ierr = c_cg_init(this%h, c_loc(params), this%prec)
+        if (ierr /= 0) return
+        this%nxp = params%nx
+        this%nyp = params%ny
+        this%nzp = params%nz
+    end function
+
+    subroutine cg_get_rhs(this, W, rhs)
+        class(cgrid), intent(inout) :: this
+        real(rkind), dimension(this%nxp, this%nyp, this%nzp, 5), intent(in), target :: W
+        real(rkind), dimension(this%nxp, this%nyp, this%nzp, 5), intent(out), target :: rhs
+        call check(c_cg_rhs(this%h, this%prec, c_loc(W), c_loc(rhs), this%stream))
+    end subroutine
+
+    ! cgrid%filter(arr, myfil, numtimes): numtimes passes (default 1) along
+    ! y, x and z with the conserved-variable filter fil, or, with
+    ! gaussian = .true., the Gaussian gfil (the reference passes this%gfil as
+    ! myfil for that); the boundary flags are the init's
+    subroutine cg_filter(this, arr, numtimes, gaussian)
+        class(cgrid), intent(inout) :: this
+        real(rkind), dimension(this%nxp, this%nyp, this%nzp), intent(inout), target :: arr
+        integer, optional, intent(in) :: numtimes
+        logical, optional, intent(in) :: gaussian
+        integer(c_int) :: nt, gs
+        nt = 1
+        if (present(numtimes)) nt = numtimes
+        gs = 0
+        if (present(gaussian)) then
+            if (gaussian) gs = 1
+        end if
+        call check(c_cg_filter(this%h, this%prec, c_loc(arr), nt, gs, this%stream))
+    end subroutine
+
+    subroutine cg_destroy(this)
+        class(cgrid), intent(inout) :: this
+        if (c_associated(this%h)) call check(c_cg_destroy(this%h))
         this%h = c_null_ptr
     end subroutine
 

@@ -22,6 +22,9 @@
 !                                   (w zero on both walls); getPressure and
 !                                   getPressureAndUpdateRHS (walls, Stokes) of
 !                                   (sin x, 0, 0) give -cos x and u = 0
+!   cgrid                           getRHS of an entropy wave (density and
+!                                   momentum equations in closed form); filter
+!                                   and the Gaussian filter keep a constant
 !   padeops_hip_decomp              decomp_info sizes, transpose_x_to_y copy
 ! Prints PASS or stops with a nonzero code.
 module shim_callbacks
@@ -58,7 +61,9 @@ program test_shim
     complex(c_double), allocatable, target :: zc(:,:,:), zE(:,:,:), zr(:,:,:)
     real(c_double), pointer, contiguous :: fd(:,:,:), dfd(:,:,:), ad(:,:,:), bd(:,:,:), ed(:,:,:)
     complex(c_double), pointer, contiguous :: cu(:,:,:), cv(:,:,:), cw(:,:,:), ct(:,:,:)
-    type(c_ptr) :: pf, pdf, pa, pb, pe, pcu, pcv, pcw, pct
+    real(c_double), allocatable, target :: q(:,:,:,:), qr(:,:,:,:)
+    real(c_double), pointer, contiguous :: Wd(:,:,:,:), Rd(:,:,:,:), Fq(:,:,:)
+    type(c_ptr) :: pf, pdf, pa, pb, pe, pcu, pcv, pcw, pct, pW, pR
     type(cd06) :: d
     type(cf90) :: flt
     type(cd06stagg) :: stg, stgnp
@@ -68,6 +73,8 @@ program test_shim
     type(spectral) :: spC, spE, spW
     type(Pade6stagg) :: p6
     type(padepoisson) :: pp
+    type(cgrid) :: cg
+    type(cgrid_params) :: cgp
     type(decomp_info) :: gpC, gpE
     integer(c_size_t) :: bytes, rb, rbE, cb, cbE
     integer :: i, j, k, ierr
@@ -459,6 +466,55 @@ program test_shim
     call p6%destroy()
     call spC%destroy()
     call spE%destroy()
+
+    ! ---------------------------------------------------------------- cgrid
+    ! entropy wave rho = 1 + 0.1 sin x, u = 0.5, p = 1 on a periodic
+    ! 32 x 16 x 16 grid (cd10, LAD on, inviscid): d(rho)/dt = -0.05 cos x,
+    ! d(rho u)/dt = -0.025 cos x, the v and w equations vanish (the artificial
+    ! conductivity acts on the energy equation only)
+    cgp%nx = 2 * m; cgp%ny = m; cgp%nz = m
+    cgp%dx = 2 * pi / (2 * m); cgp%dy = 2 * pi / m; cgp%dz = 2 * pi / m
+    ierr = cg%init(cgp)
+    if (ierr /= 0) error stop 61
+    allocate(q(2 * m, m, m, 5), qr(2 * m, m, m, 5))
+    do i = 1, 2 * m
+        x = (i - 1) * cgp%dx
+        q(i, :, :, 1) = 1.d0 + 0.1d0 * sin(x)
+        q(i, :, :, 2) = 0.5d0 * q(i, :, :, 1)
+        q(i, :, :, 3) = 0
+        q(i, :, :, 4) = 0
+        q(i, :, :, 5) = 1.d0 / (cgp%gam - 1.d0) + 0.125d0 * q(i, :, :, 1)
+    end do
+    bytes = int(8, c_size_t) * size(q)
+    pW = pdo_alloc(bytes)
+    pR = pdo_alloc(bytes)
+    call c_f_pointer(pW, Wd, [2 * m, m, m, 5])
+    call c_f_pointer(pR, Rd, [2 * m, m, m, 5])
+    call pdo_to_device(pW, c_loc(q), bytes)
+    call cg%getRHS(Wd, Rd)
+    call pdo_synchronize(c_null_ptr)
+    call pdo_to_host(c_loc(qr), pR, bytes)
+    err = 0
+    do i = 1, 2 * m
+        x = (i - 1) * cgp%dx
+        err = max(err, maxval(abs(qr(i, :, :, 1) + 0.05d0 * cos(x))), maxval(abs(qr(i, :, :, 2) + 0.025d0 * cos(x))))
+    end do
+    err = max(err, maxval(abs(qr(:, :, :, 3:4))))
+    print '(a,es10.3)', "cgrid getRHS of an entropy wave vs -0.05 cos x, -0.025 cos x, 0, 0: ", err
+    if (err > 1d-9) error stop 62
+    ! filter and the Gaussian filter keep a constant
+    q(:, :, :, 1) = 2.5d0
+    call pdo_to_device(pW, c_loc(q), int(8, c_size_t) * 2 * m * m * m)
+    call c_f_pointer(pW, Fq, [2 * m, m, m])
+    call cg%filter(Fq)
+    call cg%filter(Fq, numtimes=2, gaussian=.true.)
+    call pdo_synchronize(c_null_ptr)
+    call pdo_to_host(c_loc(qr), pW, int(8, c_size_t) * 2 * m * m * m)
+    err = maxval(abs(qr(:, :, :, 1) - 2.5d0))
+    print '(a,es10.3)', "cgrid filter (fil, then gfil twice) of a constant: ", err
+    if (err > 1d-13) error stop 63
+    call cg%destroy()
+    call pdo_release(pW); call pdo_release(pR)
 
     call pdo_release(pa); call pdo_release(pb); call pdo_release(pe)
     call pdo_release(pcu); call pdo_release(pcv); call pdo_release(pcw); call pdo_release(pct)

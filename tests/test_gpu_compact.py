@@ -70,8 +70,6 @@ def _make(scheme, n, periodic, bc1=0, bcn=0):
 @pytest.mark.parametrize("scheme,meth,oop", COLLOCATED)
 def test_periodic_fp64(cuda, scheme, meth, oop, dir, n):
     import torch
-    if scheme == "cf90" and n < 10:
-        pytest.skip("cf90 needs n >= 10")
     f = random_field(_shape(n, dir), seed=n * 10 + dir)
     o = _make(scheme, n, True)
     extra = () if scheme == "cd06" else (0, 0)
@@ -109,14 +107,15 @@ def test_cd06_nonperiodic_fp64(cuda, dir, n, bc):
     assert rel_l2(g, ref) <= 1e-12
 
 
+# the cd06 second derivative is built for periodic pencils only
+FP32_CASES = [c + (p,) for c in COLLOCATED for p in (True, False) if p or not (c[0] == "cd06" and c[1] == "d2d")]
+
+
 @pytest.mark.parametrize("prec,bar", [("f32_refine", 2e-7), ("f32_store", 2e-7), ("f32", 5e-3)])
 @pytest.mark.parametrize("dir", [1, 2, 3])
-@pytest.mark.parametrize("scheme,meth,oop", COLLOCATED)
-@pytest.mark.parametrize("periodic", [True, False])
+@pytest.mark.parametrize("scheme,meth,oop,periodic", FP32_CASES)
 def test_fp32_modes(cuda, scheme, meth, oop, dir, prec, bar, periodic):
     import torch
-    if not periodic and (scheme == "cd06" and meth == "d2d"):
-        pytest.skip("cd06 d2 is periodic only")
     n = 128
     f = random_field(_shape(n, dir, (70, 3)), seed=5 + dir).astype(np.float32).astype(np.float64)
     o = _make(scheme, n, periodic)

@@ -2,7 +2,7 @@
 (built by padeops_amd.build when amdflang is installed) drives every type of
 the shim on device arrays (cd06, cf90, cd06stagg periodic and non-periodic
 with its real and complex generics, PoissonPeriodic%init with a Get_ModKz
-callback, derivatives, filters, spectral, Pade6stagg, padepoisson and the
+callback, derivatives, filters, spectral, Pade6stagg, padepoisson, cgrid and the
 decomp_info / transpose_* module) and checks each against a closed form
 (modified wavenumbers, single Fourier modes, constants) and the reference's
 init codes."""
