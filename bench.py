@@ -355,7 +355,25 @@ def wall_les_bench(P, dev, stream, steps=3):
     out["mlp_infer"] = {"points": npts, "ms": round(mlp_ms, 3), "tflops": round(flop / mlp_ms / 1e9, 1),
                         "frac_bf16_peak": round(flop / mlp_ms / 1e9 / 2500.0, 4),
                         "gelu_evals_T_per_s": round(192.0 * npts / mlp_ms / 1e9, 3)}
-    del net, X, Y
+    del net, Y
+    # one training step (forward keeping the activations, backward, ordered
+    # gradient sum, Adam) at the closure's training batch of 65,536 points
+    B = 65536
+    tnet = P.SGSNet(B)
+    tnet.set_params(params)
+    Xb = X[:B].contiguous()
+    Tb = torch.randn((B, 6), dtype=torch.float32, device=dev)
+    for _ in range(3):
+        tnet.train_step(Xb, Tb, lr=1e-5)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        tnet.train_step(Xb, Tb, lr=1e-5)
+    e1.record()
+    torch.cuda.synchronize()
+    tr_ms = e0.elapsed_time(e1) / reps
+    out["mlp_train_step"] = {"batch": B, "ms": round(tr_ms, 4), "samples_per_s": round(B / tr_ms * 1e3, 0)}
+    del tnet, X, Xb, Tb
     return out
 
 

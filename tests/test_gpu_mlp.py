@@ -54,6 +54,34 @@ def test_forward_and_gradients(cuda, B):
     assert np.all(after[pad] == 0) and np.all(lib_before[pad] == 0)
 
 
+@pytest.mark.parametrize("B", [5, 400003])
+def test_inference_kernel(cuda, B):
+    """The inference forward (persistent workgroups, several 16-sample tiles
+    per wave at B = 400003, a partial last tile, and a single partial tile at
+    B = 5) against the oracle and against the training forward, which keeps
+    its activations and evaluates GELU through tanh."""
+    import torch
+    rng = np.random.default_rng(11)
+    params = [(W, (0.1 * rng.standard_normal(b.shape)).astype(np.float32)) for W, b in M.init_params(4)]
+    X = rng.standard_normal((B, 10)).astype(np.float32)
+    net = P.SGSNet(max(B, 64))
+    net.set_params(P.SGSNet.pack(params))
+    Xd = _dev(X, cuda)
+    Y = torch.full((B + 1, 6), 7.0, dtype=torch.float32, device=cuda)
+    assert net.forward(Xd, Y[:B]) == 0
+    Yt = torch.empty((B, 6), dtype=torch.float32, device=cuda)
+    assert net.forward_train(Xd, Yt) == 0
+    torch.cuda.synchronize()
+    Yref, _ = M.forward(params, X)
+    got = Y.cpu().numpy()
+    e_oracle = rel_l2(got[:B], Yref)
+    e_train = rel_l2(got[:B], Yt.cpu().numpy())
+    print("B", B, "vs oracle", e_oracle, "vs training forward", e_train)
+    assert e_oracle <= 2e-3 and e_train <= 2e-3
+    # nothing written past the batch
+    assert np.all(got[B] == 7.0)
+
+
 def test_adam_tracks_oracle(cuda):
     import torch
     rng = np.random.default_rng(7)
