@@ -244,9 +244,22 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
     // TX is a compile-time constant so that the LDS addresses of a thread's
     // elements are one base register plus immediate offsets
     const int ln = threadIdx.x & (TX - 1), jb = threadIdx.x / TX;
-    const long a = (long)blockIdx.x * TX + ln;
+    // XCD-aware tile order: workgroups are dispatched round-robin over the
+    // 8 XCDs in flattened order w, so XCD w % 8 takes the contiguous run of
+    // logical tiles below.  Neighbouring line groups of a row then share one
+    // L2; with an odd row pitch (the (nx/2+1, ny, nz) spectra) a group's row
+    // segment straddles two 128-byte lines, each half read by a neighbour.
+    unsigned bx, by;
+    {
+        const unsigned gx = gridDim.x, nw = gx * gridDim.y, w = blockIdx.y * gx + blockIdx.x;
+        const unsigned per = nw / 8, rem = nw % 8, xcd = w % 8, slot = w / 8;
+        const unsigned lg = xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+        bx = lg % gx;
+        by = lg / gx;
+    }
+    const long a = (long)bx * TX + ln;
     const bool act = a < na;
-    const long off = 2 * ((act ? a : 0) + (long)blockIdx.y * sb);
+    const long off = 2 * ((act ? a : 0) + (long)by * sb);
     const TS *const lin = src + off;
     TS *const lout = dst + off;
     const C2<TC> *tw = (const C2<TC> *)twL;
@@ -281,10 +294,10 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
         line_fft<TC, +1, false, L>(ldg, stg, buf, tw, jb, act);
     } else {
         // forward into LDS, multiply by the spectral factor of (kx, ky, kz) =
-        // (a, blockIdx.y, i), inverse from LDS to HBM
+        // (a, by, i), inverse from LDS to HBM
         // the factor is formed in the arithmetic of the transform (an FP64
         // division per point costs as much as the transforms in FP32)
-        const int kx = (int)a, ky = (int)blockIdx.y;
+        const int kx = (int)a, ky = (int)by;
         const bool line_only = pm.kind >= 3;  // factors of the line index alone
         const TC fxy = (act && !line_only) ? (TC)(pm.kind == 2 ? pm.ax[kx] * pm.ay[ky] : pm.ax[kx] + pm.ay[ky])
                                            : TC(0);
