@@ -235,6 +235,21 @@ __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x
 // Tile: TX = 1 << lgtx consecutive lines; thread = (line, jb), line fastest.
 // MODE 0 forward, 1 inverse, 2 forward -> Poisson multiply -> inverse.
 // ---------------------------------------------------------------------------
+// XCD-aware tile order of a 2D grid of (line group, plane) tiles: workgroups
+// are dispatched round-robin over the 8 XCDs in flattened order w, so XCD
+// w % 8 gets the contiguous run of logical tiles below (a bijection for any
+// grid).  Neighbouring line groups of a row then share one L2; with an odd
+// row pitch (the (nx/2+1, ny, nz) spectra) a group's row segment straddles
+// two 128-byte lines, each half read by a neighbour.
+__device__ __forceinline__ void xcd_tile(unsigned &bx, unsigned &by)
+{
+    const unsigned gx = gridDim.x, nw = gx * gridDim.y, w = blockIdx.y * gx + blockIdx.x;
+    const unsigned per = nw / 8, rem = nw % 8, xcd = w % 8, slot = w / 8;
+    const unsigned lg = xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+    bx = lg % gx;
+    by = lg / gx;
+}
+
 template <typename TS, typename TC, int MODE, int L, int TX>
 __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, long na, long sb, int se,
                                                      const TC *__restrict__ twL, SpecMul pm)
@@ -244,19 +259,8 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
     // TX is a compile-time constant so that the LDS addresses of a thread's
     // elements are one base register plus immediate offsets
     const int ln = threadIdx.x & (TX - 1), jb = threadIdx.x / TX;
-    // XCD-aware tile order: workgroups are dispatched round-robin over the
-    // 8 XCDs in flattened order w, so XCD w % 8 takes the contiguous run of
-    // logical tiles below.  Neighbouring line groups of a row then share one
-    // L2; with an odd row pitch (the (nx/2+1, ny, nz) spectra) a group's row
-    // segment straddles two 128-byte lines, each half read by a neighbour.
     unsigned bx, by;
-    {
-        const unsigned gx = gridDim.x, nw = gx * gridDim.y, w = blockIdx.y * gx + blockIdx.x;
-        const unsigned per = nw / 8, rem = nw % 8, xcd = w % 8, slot = w / 8;
-        const unsigned lg = xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
-        bx = lg % gx;
-        by = lg / gx;
-    }
+    xcd_tile(bx, by);
     const long a = (long)bx * TX + ln;
     const bool act = a < na;
     const long off = 2 * ((act ? a : 0) + (long)by * sb);
@@ -933,9 +937,11 @@ __global__ void __launch_bounds__(1024) k_pp_walls(TS *f, TS *w, long na, long s
     C2<TC> *tW = tF + TX * L;
     C2<TC> *twl = tW + TX * L;
     const int ln = threadIdx.x & (TX - 1), jb = threadIdx.x / TX;
-    const long a = (long)blockIdx.x * TX + ln;
+    unsigned bx, by;
+    xcd_tile(bx, by);
+    const long a = (long)bx * TX + ln;
     const bool act = a < na;
-    const long off = 2 * ((act ? a : 0) + (long)blockIdx.y * sb);
+    const long off = 2 * ((act ? a : 0) + (long)by * sb);
     TS *const lf = f + off;
     TS *const lw = w + off;
     for (int i = threadIdx.x; i < L; i += TX * (L / 8)) twl[i] = ((const C2<TC> *)twL)[i];
@@ -962,7 +968,7 @@ __global__ void __launch_bounds__(1024) k_pp_walls(TS *f, TS *w, long na, long s
     __syncthreads();
     // the pressure and the w update, 8 modes per thread
     if (act) {
-        const TC fxy = (TC)(ax[a] + ay[blockIdx.y]);
+        const TC fxy = (TC)(ax[a] + ay[by]);
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const int i = jb * 8 + r;

@@ -389,19 +389,25 @@ hipError_t launch_mlp_forward(const float *X, float *Y, const unsigned short *Wb
     if (B <= 0) return hipSuccess;
     if (!cache.H[0]) {
         // inference: as many persistent workgroups as are resident at once
-        static int resident = 0;
-        if (!resident) {
-            int dev = 0, ncu = 0, per_cu = 0;
-            hipError_t e = hipGetDevice(&dev);
-            if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        // (per device; a race on first use only stores the same value twice)
+        constexpr int MAXDEV = 64;
+        static int resident[MAXDEV] = {};
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return e;
+        int res = (dev >= 0 && dev < MAXDEV) ? resident[dev] : 0;
+        if (!res) {
+            int ncu = 0, per_cu = 0;
+            e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
             if (e == hipSuccess)
                 e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mlp_infer, 64 * INFER_WAVES, 0);
             if (e != hipSuccess) return e;
-            resident = ncu * (per_cu > 0 ? per_cu : 1);
+            res = ncu * (per_cu > 0 ? per_cu : 1);
+            if (dev >= 0 && dev < MAXDEV) resident[dev] = res;
         }
         const long ntiles = (B + 15) / 16;  // 16 samples per wave
         const long need = (ntiles + INFER_WAVES - 1) / INFER_WAVES;
-        const long nwg = need < resident ? need : resident;
+        const long nwg = need < res ? need : res;
         hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)nwg), dim3(64 * INFER_WAVES), 0, st, X, Y, Wb, P, B, ntiles);
         return hipGetLastError();
     }
