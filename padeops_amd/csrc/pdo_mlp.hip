@@ -10,6 +10,8 @@
 // second kernel sums in a fixed order, so training is deterministic.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "pdo_mlp.hpp"
 
 namespace pdo {
@@ -389,13 +391,13 @@ hipError_t launch_mlp_forward(const float *X, float *Y, const unsigned short *Wb
     if (B <= 0) return hipSuccess;
     if (!cache.H[0]) {
         // inference: as many persistent workgroups as are resident at once
-        // (per device; a race on first use only stores the same value twice)
+        // (cached per device)
         constexpr int MAXDEV = 64;
-        static int resident[MAXDEV] = {};
+        static std::atomic<int> resident[MAXDEV];
         int dev = 0;
         hipError_t e = hipGetDevice(&dev);
         if (e != hipSuccess) return e;
-        int res = (dev >= 0 && dev < MAXDEV) ? resident[dev] : 0;
+        int res = (dev >= 0 && dev < MAXDEV) ? resident[dev].load(std::memory_order_relaxed) : 0;
         if (!res) {
             int ncu = 0, per_cu = 0;
             e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -403,7 +405,7 @@ hipError_t launch_mlp_forward(const float *X, float *Y, const unsigned short *Wb
                 e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mlp_infer, 64 * INFER_WAVES, 0);
             if (e != hipSuccess) return e;
             res = ncu * (per_cu > 0 ? per_cu : 1);
-            if (dev >= 0 && dev < MAXDEV) resident[dev] = res;
+            if (dev >= 0 && dev < MAXDEV) resident[dev].store(res, std::memory_order_relaxed);
         }
         const long ntiles = (B + 15) / 16;  // 16 samples per wave
         const long need = (ntiles + INFER_WAVES - 1) / INFER_WAVES;
