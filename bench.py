@@ -290,7 +290,8 @@ def wall_les_bench(P, dev, stream, steps=3):
     """One SSP-RK45 step of the wall-modelled LES on the 512 x 512 x 256 box
     (periodic x, y of side 2 pi; walls in z, dz = 2 pi / 512: the Moeng wall
     model below, a slip wall above, dpFdx = 1; log-law mean profile with
-    z0 = 1e-4 plus the seeded turbulence; FP32+refine), with the learned
+    z0 = 1e-4 plus the seeded turbulence of spectrum k^4 exp(-2 (k/4)^2);
+    FP32+refine), with the learned
     closure at every stage and the dynamic Smagorinsky model where its
     dissipation is negative, using the network trained on the solver's own
     256^3 DNS (results/closure_c4_dns256_params.npy).  Reports ms per step
@@ -310,7 +311,7 @@ def wall_les_bench(P, dev, stream, steps=3):
     g.init(nx, nx, nz, 2 * math.pi, 2 * math.pi, Lz, Re=0.0, scheme=2, CFL=0.5,
            sgs=dict(model="smagorinsky", cglobal=0.01, wall_model="moeng", z0=1e-4),
            walls=dict(botWall=3, topWall=2, dpFdx=1.0), prec="f32_refine")
-    uC, _, wE = wall_bounded(nx, nx, nz, 8.0, 1234, torch.float32, dev)
+    uC, _, wE = wall_bounded(nx, nx, nz, 4.0, 1234, torch.float32, dev)
     wE.view(nz + 1, -1)[0].zero_()
     wE.view(nz + 1, -1)[nz].zero_()
     g.set_velocity(uC, torch.zeros_like(uC), wE)
@@ -379,7 +380,8 @@ def wall_les_bench(P, dev, stream, steps=3):
 
 def wall_box(P, dev, stream, steps):
     """The wall-bounded box of 512 x 512 x 256 cells (periodic x, y; walls in
-    z; log-law mean profile plus the seeded turbulence): time per launch of
+    z; log-law mean profile plus the seeded turbulence of spectrum
+    k^4 exp(-2 (k/4)^2)): time per launch of
     each operator in FP32+refine and FP64 (HIP events on the launch stream),
     the achieved algorithmic GB/s (read the input field, write the output
     field), and the FP32+refine error against FP64."""
@@ -389,7 +391,7 @@ def wall_box(P, dev, stream, steps):
     from padeops_amd.synthetic import wall_bounded
     nx, nz = 512, 256
     dz = 2 * math.pi / nx
-    F64 = dict(zip(("uC", "uE", "wE"), wall_bounded(nx, nx, nz, 8.0, 1234, torch.float64, dev)))
+    F64 = dict(zip(("uC", "uE", "wE"), wall_bounded(nx, nx, nz, 4.0, 1234, torch.float64, dev)))
     F32 = {k: v.float() for k, v in F64.items()}
     c10z = P.cd10()
     c10z.init(nz, dz, False, 0, 0)

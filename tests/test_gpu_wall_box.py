@@ -1,7 +1,7 @@
 """The wall-bounded box: 512 x 512 x 256 cells, periodic in x and y (side
 2 pi), non-periodic in z (dz = dx, walls at z = 0 and z = pi), with the
 log-law mean profile U(z) = ln(z / 1e-4) / 0.4 plus the seeded isotropic
-turbulence (E(k) ~ k^4 exp(-2 (k/8)^2), rms 1); see
+turbulence (E(k) ~ k^4 exp(-2 (k/4)^2), rms 1); see
 padeops_amd.synthetic.wall_bounded.
 
 Operators, as the wall-normal direction of the igrid layout uses them:
@@ -55,7 +55,7 @@ def _fields(cuda):
     if "uC" not in _F:
         import torch
         from padeops_amd.synthetic import wall_bounded
-        _F["uC"], _F["uE"], _F["wE"] = wall_bounded(NX, NX, NZ, 8.0, 1234, torch.float64, cuda)
+        _F["uC"], _F["uE"], _F["wE"] = wall_bounded(NX, NX, NZ, 4.0, 1234, torch.float64, cuda)
     return _F
 
 
