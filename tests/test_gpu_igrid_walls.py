@@ -6,6 +6,7 @@ pressure gradient.
   * parity: two steps against the oracle's restatement (oracle/igrid.py with
     walls), FP64, relative L2 <= 1e-11 for the spectra and the physical
     velocity; w is zero on both walls;
+  * FP32 storage with refined solves against FP64 after two steps, 1e-5;
   * laminar Poiseuille flow between no-slip walls, from rest: the profile
     converges to the discrete steady state of the oracle's second-derivative
     operator (to 2e-6) and so to the parabola Re dpFdx z (Lz - z) / 2 within
@@ -30,12 +31,17 @@ from oracle.spectral import pade6_apply
 pytestmark = pytest.mark.gpu
 
 L = 2 * np.pi
+# Smagorinsky's multiplier (Cs Delta)^2, Cs = 0.12, Delta = (dx dy dz)^(1/3) of
+# the 32 x 16 x 32 box of height 1: the time step follows the advective CFL
+# only (igrid.F90:1374-1400), so a larger constant makes the explicit
+# wall-normal diffusion of the eddy viscosity unstable
+CG_WALLS = (0.12 * (L / 32 * L / 16 * 1.0 / 32) ** (1.0 / 3.0)) ** 2
 CASES = {
     "noslip-noslip-rk45": dict(nx=16, ny=16, nz=32, scheme=2, Re=100.0, walls=dict(botWall=1, topWall=1, dpFdx=1.0),
                                sgs=None),
     "wallmodel-slip-smag-rk45": dict(nx=32, ny=16, nz=32, scheme=2, Re=0.0,
                                      walls=dict(botWall=3, topWall=2, dpFdx=1.0),
-                                     sgs=dict(model="smagorinsky", cglobal=0.01, wall_model="moeng", z0=1e-4)),
+                                     sgs=dict(model="smagorinsky", cglobal=CG_WALLS, wall_model="moeng", z0=1e-4)),
     "slip-slip-stokes-rk3": dict(nx=16, ny=16, nz=16, scheme=1, Re=200.0,
                                  walls=dict(botWall=2, topWall=2, dpFdx=0.5, stokes_pressure=True), sgs=None),
 }
@@ -106,6 +112,35 @@ def test_walls_steps_fp64(cuda, case):
         assert e <= 1e-11, k
     wg = _host(g.field("w"), shapes["w"])
     assert np.abs(wg[:, :, 0]).max() <= 1e-13 and np.abs(wg[:, :, nz]).max() <= 1e-13
+    # a stable run: the velocity stays of the initial field's size
+    assert np.abs(_host(g.field("u"), shapes["u"])).max() <= 2 * np.abs(u).max()
+
+
+@pytest.mark.parametrize("case", ["noslip-noslip-rk45", "wallmodel-slip-smag-rk45"])
+def test_walls_fp32_refine(cuda, case):
+    """FP32 storage with refined solves against FP64, two steps from the same
+    initial field (the FP32 run takes it rounded): every spectrum and
+    physical field to 1e-5 relative L2, w zero on both walls."""
+    import torch
+    c = CASES[case]
+    nx, ny, nz, Lz = c["nx"], c["ny"], c["nz"], 1.0
+    mean = lambda z: 2.0 + np.log1p(20 * z)  # noqa: E731
+    u, v, w = _initial(nx, ny, nz, Lz, mean)
+    nxh = nx // 2 + 1
+    shapes = dict(uhat=(nxh, ny, nz), vhat=(nxh, ny, nz), what=(nxh, ny, nz + 1), u=(nx, ny, nz), v=(nx, ny, nz),
+                  w=(nx, ny, nz + 1))
+    out = {}
+    for prec, dt in (("f64", torch.float64), ("f32_refine", torch.float32)):
+        g = _gpu_igrid(c, Lz, prec=prec)
+        assert g.set_velocity(_dev(u, cuda, dt), _dev(v, cuda, dt), _dev(w, cuda, dt)) == 0
+        assert g.step(2) == 0
+        out[prec] = {k: _host(g.field(k), sh) for k, sh in shapes.items()}
+    for k in shapes:
+        e = rel_l2(out["f32_refine"][k], out["f64"][k])
+        print(case, k, e)
+        assert e <= 1e-5, k
+    w32 = out["f32_refine"]["w"]
+    assert np.abs(w32[:, :, 0]).max() == 0 and np.abs(w32[:, :, nz]).max() == 0
 
 
 def test_poiseuille_converges_to_the_discrete_steady_state(cuda):
