@@ -82,6 +82,8 @@ def test_igrid_steps_fp64(cuda, case):
         e = rel_l2(out[k], r)
         print(k, e)
         assert e <= 1e-11, k
+    # a stable run: the velocity stays of the initial field's size
+    assert np.abs(out["u"]).max() <= 2 * np.abs(_initial(nx, ny, nz)[0]).max()
     div = ref.pp.divergence(out["uhat"], out["vhat"], out["what"])
     print("max |div| / max |u|", np.abs(div).max() / np.abs(out["u"]).max())
     assert np.abs(div).max() <= 1e-10 * np.abs(out["u"]).max()
