@@ -302,9 +302,10 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
         // the factor is formed in the arithmetic of the transform (an FP64
         // division per point costs as much as the transforms in FP32)
         const int kx = (int)a, ky = (int)by;
-        const bool line_only = pm.kind >= 3;  // factors of the line index alone
-        const TC fxy = (act && !line_only) ? (TC)(pm.kind == 2 ? pm.ax[kx] * pm.ay[ky] : pm.ax[kx] + pm.ay[ky])
-                                           : TC(0);
+        const bool line_only = pm.kind == 3 || pm.kind == 4;  // factors of the line index alone
+        const TC fxy = (act && !line_only)
+                           ? (TC)(pm.kind == 5 ? pm.ax[kx] : pm.kind == 2 ? pm.ax[kx] * pm.ay[ky] : pm.ax[kx] + pm.ay[ky])
+                           : TC(0);
         const TC scale = (TC)pm.scale;
         auto sts = [&](int i, C2<TC> v) {
             TC fac;
@@ -317,7 +318,7 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const TS *src, TS *dst, lo
             }
             if (pm.kind == 4) {
                 fac = az;
-            } else if (pm.kind == 2) {
+            } else if (pm.kind == 2 || pm.kind == 5) {
                 fac = fxy * az;
             } else {
                 const TC ks = fxy + az;
@@ -887,6 +888,37 @@ static hipError_t run_fft2_t(const PoissonPlanDev &pl, const TS *in, TS *out, in
     x_fft<TS, TC>(L, false, in, out, (long)ny * nplanes, tw.x, tw.xn, (TC)1, nxh, 0, st);
     strided_fft<TS, TC, 0>(ny, out, out, (long)nxh, (long)nxh * ny, (long)nxh, tw.y, pm, nplanes, st);
     return hipGetLastError();
+}
+
+// 2D spectral filtering of real (nx, ny, nplanes) data by the separable
+// factor t1(kx) t2(ky) (host-side 0/1 cutoffs, device arrays of nx/2+1 and
+// ny): the x r2c pass into spec, then one fused pass per y line (forward
+// transform, the factor, inverse transform, in place), then the x c2r pass
+// into out.  Two passes fewer than fft2, a pointwise multiply and ifft2.
+template <typename TS, typename TC>
+static hipError_t run_filter2_t(const PoissonPlanDev &pl, const TS *in, TS *spec, TS *out, int nplanes,
+                                const double *t1, const double *t2, hipStream_t st)
+{
+    const int nx = pl.nx, ny = pl.ny, nxh = nx / 2 + 1, L = nx / 2;
+    const Tw<TC> tw(pl);
+    x_fft<TS, TC>(L, false, in, spec, (long)ny * nplanes, tw.x, tw.xn, (TC)1, nxh, 0, st);
+    // kind 5: factor t1[kx] t2[ky] of the line (kx) and element (ky) indices
+    const SpecMul pm{t1, nullptr, t2, 5, 1.0};
+    strided_fft<TS, TC, 2>(ny, spec, spec, (long)nxh, (long)nxh * ny, (long)nxh, tw.y, pm, nplanes, st);
+    const TC scale = (TC)(1.0 / ((double)L * ny));
+    x_fft<TS, TC>(L, true, spec, out, (long)ny * nplanes, tw.x, tw.xn, scale, nxh, 0, st);
+    return hipGetLastError();
+}
+
+hipError_t run_filter2(const PoissonPlanDev &pl, int mode, const void *in, void *spec, void *out, int nplanes,
+                       const double *t1, const double *t2, hipStream_t st)
+{
+    switch (mode) {
+    case 0: return run_filter2_t<double, double>(pl, (const double *)in, (double *)spec, (double *)out, nplanes, t1, t2, st);
+    case 1: return run_filter2_t<float, float>(pl, (const float *)in, (float *)spec, (float *)out, nplanes, t1, t2, st);
+    case 2: return run_filter2_t<float, double>(pl, (const float *)in, (float *)spec, (float *)out, nplanes, t1, t2, st);
+    }
+    return hipErrorInvalidValue;
 }
 
 template <typename TS, typename TC>

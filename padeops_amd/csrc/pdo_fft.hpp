@@ -44,6 +44,11 @@ hipError_t run_poisson_dx(const PoissonPlanDev &pl, int mode, const void *div, c
 // 2D (x r2c, y c2c) forward transform of each of `nplanes` (nx, ny) planes into
 // an unpadded (nx/2+1, ny, nplanes) spectrum; unnormalized.
 hipError_t run_fft2(const PoissonPlanDev &pl, int mode, const void *in, void *out, int nplanes, hipStream_t st);
+// filtering of real (nx, ny, nplanes) data by t1(kx) t2(ky) (device arrays of
+// nx/2+1 and ny): r2c into spec (a half spectrum of scratch), one fused
+// forward-multiply-inverse y pass in place, c2r into out (out may be in)
+hipError_t run_filter2(const PoissonPlanDev &pl, int mode, const void *in, void *spec, void *out, int nplanes,
+                       const double *t1, const double *t2, hipStream_t st);
 // inverse of run_fft2 scaled by 1/(nx ny); `in` is not modified (work holds
 // the y-inverse planes); oddball != 0 drops the kx = nx/2 modes.
 hipError_t run_ifft2(const PoissonPlanDev &pl, int mode, const void *in, void *work, void *out, int nplanes,

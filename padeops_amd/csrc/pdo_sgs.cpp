@@ -118,18 +118,16 @@ int pdo_dynproc_compute(pdo_dynproc h, int prec, const void *u, const void *v, c
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     int rc;
-    // TestFilter_Real_to_Real (standardDynamicProcedure.F90:165-179)
+    // TestFilter_Real_to_Real (standardDynamicProcedure.F90:165-179): fft,
+    // the cutoff, ifft, with the y transforms and the cutoff in one pass
     auto tf = [&](const void *in, void *out) -> int {
-        if ((e = run_fft2(sp.plan, mode, in, T, sp.nz, st)) != hipSuccess) return hip_fail(e, "test filter");
-        if ((e = launch_mask2(mode, T, h->d_t1, h->d_t2, sp.nxh, sp.ny, nc, st)) != hipSuccess)
+        if ((e = run_filter2(sp.plan, mode, in, T, out, sp.nz, h->d_t1, h->d_t2, st)) != hipSuccess)
             return hip_fail(e, "test filter");
-        return pdo_spectral_ifft(static_cast<pdo_spectral>(h->sp), prec, T, out, 0, stream);
+        return 0;
     };
-    // TestFilter_Cmplx_to_Real (:195-207)
+    // TestFilter_Cmplx_to_Real (:195-207): the cutoff of a copy of fhat, ifft
     auto tfc = [&](const void *fhat, void *out) -> int {
-        if ((e = hipMemcpyAsync(T, fhat, spec, hipMemcpyDeviceToDevice, st)) != hipSuccess)
-            return hip_fail(e, "test filter");
-        if ((e = launch_mask2(mode, T, h->d_t1, h->d_t2, sp.nxh, sp.ny, nc, st)) != hipSuccess)
+        if ((e = launch_mask2_oop(mode, fhat, T, h->d_t1, h->d_t2, sp.nxh, sp.ny, nc, st)) != hipSuccess)
             return hip_fail(e, "test filter");
         return pdo_spectral_ifft(static_cast<pdo_spectral>(h->sp), prec, T, out, 0, stream);
     };

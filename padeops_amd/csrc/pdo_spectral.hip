@@ -40,17 +40,18 @@ __global__ void __launch_bounds__(256) k_mtimes_ik(const TS *in, TS *out, const 
     }
 }
 
-// f *= m1[i] m2[j]
+// f = src m1[i] m2[j] (src may be f)
 template <typename TS>
-__global__ void __launch_bounds__(256) k_mask2(TS *f, const double *__restrict__ m1, const double *__restrict__ m2,
-                                              int nxh, int ny, long n)
+__global__ void __launch_bounds__(256) k_mask2(const TS *src, TS *f, const double *__restrict__ m1,
+                                              const double *__restrict__ m2, int nxh, int ny, long n)
 {
     using V = typename CV<TS>::V;
+    const V *q = (const V *)src;
     V *p = (V *)f;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
         const long i = t % nxh, j = (t / nxh) % ny;
         const TS g = (TS)(m1[i] * m2[j]);
-        const V x = p[t];
+        const V x = q[t];
         p[t] = V{x.x * g, x.y * g};
     }
 }
@@ -126,10 +127,18 @@ hipError_t launch_mtimes_ik(int mode, const void *in, void *out, const double *k
 hipError_t launch_mask2(int mode, void *f, const double *m1, const double *m2, int nxh, int ny, long n,
                         hipStream_t st)
 {
+    return launch_mask2_oop(mode, f, f, m1, m2, nxh, ny, n, st);
+}
+
+hipError_t launch_mask2_oop(int mode, const void *src, void *f, const double *m1, const double *m2, int nxh, int ny,
+                            long n, hipStream_t st)
+{
     if (mode == 0)
-        hipLaunchKernelGGL((k_mask2<double>), dim3(grid_for(n)), dim3(256), 0, st, (double *)f, m1, m2, nxh, ny, n);
+        hipLaunchKernelGGL((k_mask2<double>), dim3(grid_for(n)), dim3(256), 0, st, (const double *)src, (double *)f,
+                           m1, m2, nxh, ny, n);
     else
-        hipLaunchKernelGGL((k_mask2<float>), dim3(grid_for(n)), dim3(256), 0, st, (float *)f, m1, m2, nxh, ny, n);
+        hipLaunchKernelGGL((k_mask2<float>), dim3(grid_for(n)), dim3(256), 0, st, (const float *)src, (float *)f, m1,
+                           m2, nxh, ny, n);
     return hipGetLastError();
 }
 

@@ -15,6 +15,9 @@ hipError_t launch_mtimes_ik(int mode, const void *in, void *out, const double *k
                             hipStream_t st);
 hipError_t launch_mask2(int mode, void *f, const double *m1, const double *m2, int nxh, int ny, long n,
                         hipStream_t st);
+// f = src m1[i] m2[j] out of place
+hipError_t launch_mask2_oop(int mode, const void *src, void *f, const double *m1, const double *m2, int nxh, int ny,
+                            long n, hipStream_t st);
 hipError_t launch_div_spec(int mode, const void *u, const void *v, const void *d, void *f, const double *k1,
                            const double *k2, int nxh, int ny, long n, hipStream_t st);
 hipError_t launch_proj_update(int mode, void *u, void *v, void *w, const void *p, const void *dp, const double *k1,
