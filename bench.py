@@ -289,7 +289,7 @@ def cgrid_bench(P, dev, stream, steps):
 def wall_les_bench(P, dev, stream, steps=3):
     """One SSP-RK45 step of the wall-modelled LES on the 512 x 512 x 256 box
     (periodic x, y of side 2 pi; walls in z, dz = 2 pi / 512: the Moeng wall
-    model below, a slip wall above, dpFdx = 1; log-law mean profile with
+    model below, a slip wall above, dpFdx = 1, the Stokes pressure; log-law mean profile with
     z0 = 1e-4 plus the seeded turbulence of spectrum k^4 exp(-2 (k/4)^2);
     FP32+refine), with the learned
     closure at every stage and the dynamic Smagorinsky model where its

@@ -500,7 +500,10 @@ typedef struct {
      * (with sgs.wall_model); topWall 1 or 2 (get_boundary_conditions_stencil
      * igrid.F90:5148-5220); dpFdx the mean pressure gradient of
      * addExtraForcingTerm (:1759-1765, 0: none); stokes_pressure the
-     * PadePoisson Stokes pressure (computeStokesPressure, Lz) */
+     * PadePoisson Stokes pressure (computeStokesPressure, Lz; igrid's
+     * namelist default is on, igrid.F90:330: without it the projection
+     * zeroes the wall w each stage's update leaves there, and the wall cells
+     * keep that divergence) */
     int walls, botWall, topWall;
     double dpFdx;
     int stokes_pressure;

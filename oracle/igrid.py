@@ -57,7 +57,10 @@ class Igrid:
         self.Re, self.scheme, self.CFL, self.dt_fixed = Re, scheme, CFL, dt
         self.sgs = sgs            # dict of rhs_sgs keyword arguments (z0 for the wall model), or None
         self.forcing = forcing    # dict(kmin, kmax, nwaves, eps, seed), or None
-        self.walls = walls        # dict(botWall, topWall, dpFdx, stokes_pressure), or None
+        # dict(botWall, topWall, dpFdx, stokes_pressure), or None; the Stokes
+        # pressure is on unless stokes_pressure is False, as igrid's namelist
+        # default ComputeStokesPressure = .true. (igrid.F90:330)
+        self.walls = walls
         self.per = walls is None
         self.spC = Spectral(nx, ny, nz, self.dx, self.dy, self.dz, periodic_z=self.per)
         self.spE = Spectral(nx, ny, nz + 1, self.dx, self.dy, self.dz)
@@ -68,7 +71,7 @@ class Igrid:
             self.bc = (1, 1, 1, 1, -1, -1, -1, -1, -1, -1, 0, 0)
         else:
             self.pp = PadePoissonWalls(self.spC, self.dx, self.dy, self.dz,
-                                       Lz=Lz if walls.get("stokes_pressure") else None)
+                                       Lz=Lz if walls.get("stokes_pressure", True) else None)
             bc = [1, 1, 1, 1, -1, -1, -1, -1, -1, -1, 0, 0]
             for side, wall in enumerate((walls["botWall"], walls["topWall"])):
                 # (uBC = vBC, wBC, dUdzBC = dVdzBC, dWdzBC) of each wall kind

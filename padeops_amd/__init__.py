@@ -1006,7 +1006,10 @@ class igrid:
         """sgs: dict(model, explicit_edge, camd, cglobal, wall_model, z0) or
         None; forcing: dict(kmin, kmax, nwaves, eps, seed) or None; walls:
         dict(botWall, topWall, dpFdx, stokes_pressure) for walls in z (botWall
-        1 no-slip, 2 slip, 3 wall model with sgs wall_model "moeng"), or None
+        1 no-slip, 2 slip, 3 wall model with sgs wall_model "moeng";
+        stokes_pressure defaults to True, igrid's ComputeStokesPressure
+        default: without it the projection zeroes the wall w that each stage's
+        update leaves there, and the wall cells keep that divergence), or None
         for periodic z."""
         p = _lib.IgridParams()
         p.nx, p.ny, p.nz, p.Lx, p.Ly, p.Lz = nx, ny, nz, Lx, Ly, Lz
@@ -1027,7 +1030,7 @@ class igrid:
             p.walls = 1
             p.botWall, p.topWall = int(walls.get("botWall", 1)), int(walls.get("topWall", 1))
             p.dpFdx = float(walls.get("dpFdx", 0.0))
-            p.stokes_pressure = int(bool(walls.get("stokes_pressure", False)))
+            p.stokes_pressure = int(bool(walls.get("stokes_pressure", True)))
         self.shape = (nx, ny, nz)
         self.prec = prec
         return _check(lib().pdo_igrid_init(ctypes.byref(self._h), ctypes.cast(ctypes.pointer(p), ctypes.c_void_p),

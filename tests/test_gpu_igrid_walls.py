@@ -209,3 +209,50 @@ def test_wall_modelled_les_with_the_learned_closure(cuda):
     # its dissipation comes out negative at nearly every point, so the
     # dynamic model carries the run
     assert nf > 0
+
+
+def test_wall_modelled_step_full_size(cuda):
+    """The wall-modelled box at its full size (512 x 512 x 256 cells, dz = dx,
+    Moeng wall model below, slip wall above, dpFdx = 1, Smagorinsky with
+    Cs = 0.12, FP32 + refinement) from the log law plus the C2-spectrum
+    turbulence, two SSP-RK45 steps: the fields stay finite and of the initial
+    size, w is zero on both walls, and the divergence the step's projection
+    leaves is at the FP32 rounding level (1e-7 of max |u| / dx).  This needs
+    the Stokes pressure (on by default, as igrid's ComputeStokesPressure):
+    without it the projection zeroes the wall w each stage's update leaves,
+    and the wall cells keep a divergence of about 1e-4 of that scale."""
+    import torch
+    from padeops_amd.synthetic import wall_bounded
+    nx, nz = 512, 256
+    dx = 2 * np.pi / nx
+    Lz = nz * dx
+    cs = (0.12 * dx) ** 2
+    c = dict(nx=nx, ny=nx, nz=nz, scheme=2, Re=0.0, walls=dict(botWall=3, topWall=2, dpFdx=1.0),
+             sgs=dict(model="smagorinsky", cglobal=cs, wall_model="moeng", z0=1e-4))
+    g = _gpu_igrid(c, Lz, prec="f32_refine")
+    uC, _, wE = wall_bounded(nx, nx, nz, 4.0, 1234, torch.float32, cuda)
+    wE.view(nz + 1, -1)[0].zero_()
+    wE.view(nz + 1, -1)[nz].zero_()
+    u0 = float(uC.abs().max())
+    assert g.set_velocity(uC, torch.zeros_like(uC), wE) == 0
+    del uC, wE
+    print("case wall-modelled 512x512x256 f32_refine", flush=True)
+    assert g.step(2) == 0
+    torch.cuda.synchronize()
+    u = g.field("u")
+    w = g.field("w").view(nz + 1, -1)
+    assert bool(torch.isfinite(u).all()) and bool(torch.isfinite(w).all())
+    umax = float(u.abs().max())
+    assert umax <= 2 * u0
+    assert float(w[0].abs().max()) == 0 and float(w[nz].abs().max()) == 0
+    sp = P.spectral()
+    assert sp.init(nx, nx, nz, dx, dx, dx, init_periodicInZ=False) == 0
+    pp = P.padepoisson()
+    assert pp.init(sp, dx, PeriodicInZ=False) == 0
+    div = torch.empty(nx * nx * nz, dtype=torch.float32, device=cuda)
+    assert pp.DivergenceCheck(g.field("uhat"), g.field("vhat"), g.field("what"), div, prec="f32_refine") == 0
+    torch.cuda.synchronize()
+    dmax = float(div.abs().max())
+    print("t", g.time(), "max |u|", umax, "max |div|", dmax, "scale", umax / dx)
+    # measured 2e-8 of max |u| / dx (Stokes pressure on, the default)
+    assert dmax <= 1e-7 * umax / dx
