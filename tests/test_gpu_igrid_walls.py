@@ -42,6 +42,9 @@ CASES = {
     "wallmodel-slip-smag-rk45": dict(nx=32, ny=16, nz=32, scheme=2, Re=0.0,
                                      walls=dict(botWall=3, topWall=2, dpFdx=1.0),
                                      sgs=dict(model="smagorinsky", cglobal=CG_WALLS, wall_model="moeng", z0=1e-4)),
+    "noslip-noslip-nostokes-rk45": dict(nx=16, ny=16, nz=32, scheme=2, Re=100.0,
+                                        walls=dict(botWall=1, topWall=1, dpFdx=1.0, stokes_pressure=False),
+                                        sgs=None),
     "slip-slip-stokes-rk3": dict(nx=16, ny=16, nz=16, scheme=1, Re=200.0,
                                  walls=dict(botWall=2, topWall=2, dpFdx=0.5, stokes_pressure=True), sgs=None),
 }
