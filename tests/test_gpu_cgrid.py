@@ -133,3 +133,56 @@ def test_cgrid_filter(cuda, case):
         e = rel_l2(got, ref.filter(f, "gaussian" if gaussian else "cf90", times))
         print(case, gaussian, times, e)
         assert e <= 1e-12
+
+
+def _state_at(shape, seed):
+    """the smooth state of _state on another grid of the same box"""
+    L = (2 * np.pi, 1.6 * np.pi, 1.2 * np.pi)
+    X = np.meshgrid(*[np.arange(shape[i]) * (L[i] / shape[i]) for i in range(3)], indexing="ij")
+    rng = np.random.default_rng(seed)
+
+    def smooth():
+        a = rng.uniform(-1, 1, 6)
+        return (a[0] * np.sin(X[0] + a[1]) * np.cos(2 * X[1] * 1.25) + a[2] * np.cos(X[2] * 5 / 3 + X[1] * 1.25)
+                + a[3] * np.sin(2 * X[0] - X[2] * 5 / 3) + a[4] * np.cos(X[1] * 1.25 + a[5]))
+    rho = 1.0 + 0.15 * smooth()
+    u, v, w = 0.3 * smooth(), 0.3 * smooth(), 0.3 * smooth()
+    p = 1.0 + 0.1 * smooth()
+    e = p / (0.4 * rho)
+    W = np.stack([rho, rho * u, rho * v, rho * w, rho * (e + 0.5 * (u * u + v * v + w * w))], axis=-1)
+    return W, tuple(L[i] / shape[i] for i in range(3))
+
+
+def test_cgrid_fp32_error_is_not_the_artificial_terms(cuda):
+    """Where the FP32+refine RHS error comes from, at 128^3: it is the same
+    with the artificial coefficients off (Cmu = Cbeta = Ckap = 0) as with them
+    on (measured 9.95e-6 and 9.94e-6), so it is not the LAD terms' fourth
+    derivatives.  It comes from differentiating FP32-stored intermediates
+    (the fluxes, and the viscous stresses and heat flux, which are
+    derivatives of stored first derivatives): their storage rounding is
+    lifted by the grid-scale wavenumber, so the error grows with the grid
+    (1.3e-6 at 32 x 24 x 16, 1e-5 at 128^3)."""
+    import torch
+    shape = (128, 128, 128)
+    W, d = _state_at(shape, 4)
+    W32 = W.astype(np.float32).astype(np.float64)
+    errs = {}
+    for lad in (True, False):
+        kw = dict(CASES["periodic-viscous"])
+        if not lad:
+            kw.update(Cmu=0.0, Cbeta=0.0, Ckap=0.0)
+        out = {}
+        for p in ("f64", "f32_refine"):
+            cg = P.cgrid()
+            assert cg.init(shape, d, prec=p, **kw) == 0
+            dt = torch.float64 if p == "f64" else torch.float32
+            Wd = _dev(W32, cuda, dt)
+            rhs = torch.empty_like(Wd)
+            assert cg.getRHS(Wd, rhs) == 0
+            torch.cuda.synchronize()
+            out[p] = rhs.double().cpu().numpy().reshape(shape + (5,), order="F")
+            del cg, Wd, rhs
+        errs[lad] = max(rel_l2(out["f32_refine"][..., c], out["f64"][..., c]) for c in range(5))
+    print("128^3 FP32+refine vs FP64, LAD on", errs[True], "off", errs[False])
+    assert errs[True] <= 2e-5
+    assert abs(errs[True] - errs[False]) <= 0.1 * errs[False]
