@@ -334,8 +334,20 @@ def wall_les_bench(P, dev, stream, steps=3):
     ms = e0.elapsed_time(e1) / steps
     u = g.field("u")
     finite = bool(torch.isfinite(u).all().item())
+    # the divergence the last projection leaves (DivergenceCheck on the
+    # state's spectra), against the scale max |u| / dx
+    sp = P.spectral()
+    sp.init(nx, nx, nz, 2 * math.pi / nx, 2 * math.pi / nx, dz, init_periodicInZ=False)
+    pp = P.padepoisson()
+    pp.init(sp, dz, PeriodicInZ=False)
+    div = torch.empty(nx * nx * nz, dtype=torch.float32, device=dev)
+    pp.DivergenceCheck(g.field("uhat"), g.field("vhat"), g.field("what"), div, prec="f32_refine")
+    umax = float(u.abs().max())
+    div_rel = float(div.abs().max()) / (umax / dz)
+    del sp, pp, div
     out = {"ms_per_step": round(ms, 2), "steps_per_s": round(1000.0 / ms, 3), "stages_per_step": 5,
            "prec": "f32_refine", "closure": "learned + dynamic fallback", "finite": finite,
+           "max_div_over_umax_per_dx": float("%.3e" % div_rel),
            "fallback_share": round(cl.fallback_count() / npts, 4), "t": round(g.time()[0], 5)}
     del g, cl
     torch.cuda.empty_cache()
