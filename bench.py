@@ -20,7 +20,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM = 8000.0  # GB/s, MI355X HBM3E
-WORDS_PER_POINT = 65  # algorithmic words per RHS evaluation
+# algorithmic words per point per RHS evaluation: 18 gradient passes, 18 filter
+# passes, 8 divergence, 6 + 6 FFT passes, 9 pressure-gradient subtraction
+WORDS_PER_POINT = 65
 
 
 def turbulence(n, k0, seed, dtype, device):
@@ -52,7 +54,7 @@ def main():
                     help="run only this secondary workload and print its JSON (the wall-modelled LES step)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+        print("bench.py measures one GPU: the library has no multi-GPU path (sharding is a design only)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -533,7 +533,7 @@ int pdo_igrid_set_closure(pdo_igrid h, int kind, pdo_closure c, double delta, do
 int pdo_igrid_destroy(pdo_igrid h);
 
 /* ---- learned SGS closure -----------------------------------------------------
- * No counterpart in the reference: the pointwise MLP closure of this build
+ * No counterpart in the reference: the pointwise MLP closure of this library
  * (10 inputs -> 64 -> 64 -> 64 -> 6, GELU), trained and run on the device
  * with BF16 weights / activations and FP32 accumulation on the matrix cores,
  * FP32 master parameters and Adam.  Parameters are one FP32 vector of
@@ -562,7 +562,7 @@ int pdo_mlp_backward(pdo_mlp h, const float *dY, long B, void *stream);
 int pdo_mlp_destroy(pdo_mlp h);
 
 /* ---- learned closure: invariant features and tensor-basis stress -------------
- * No counterpart in the reference (its judge is the eddy-viscosity family of
+ * No counterpart in the reference (its baseline is the eddy-viscosity family of
  * src/incompressible/sgs_models).  From a velocity-gradient field duidxj
  * (npts, 9 components npts apart, igrid's order du_i/dx_j at 3 i + j) and its
  * scale g = sqrt(mean |A|^2) the closure forms a = A / g, s = sym(a) - tr/3 I,

@@ -1,6 +1,6 @@
 """TEST INFRASTRUCTURE ONLY (the checker): numpy restatement of the learned
 closure's pointwise algebra (padeops_amd/csrc/pdo_closure.hip).  There is no
-reference counterpart: the closure is this build's own part; the pins in
+reference counterpart: the closure is this library's own part; the pins in
 tests/test_oracle_closure.py fix it by mathematics (frame invariance, the
 eigenvalues of numpy.linalg.eigvalsh, the gradient model as an exact
 combination of the basis, finite differences of the loss).

@@ -178,7 +178,7 @@ class LesFeatures:
         self.cl = closure
         self.delta = delta_of(m)
         self.dratio = self.delta / d0
-        # the dynamic procedure of the judge: its test filter acts in x and y
+        # the dynamic procedure of the comparison: its test filter acts in x and y
         # only, through a spectral object of this grid without the periodic-z
         # masks (pdo_dynproc refuses those, as InitTestFilter does)
         dx = L / m

@@ -1,5 +1,5 @@
 // Learned SGS closure: pointwise features, stress and training loss of the
-// tensor-basis MLP (no counterpart in the reference; its judge is the
+// tensor-basis MLP (no counterpart in the reference; its baseline is the
 // eddy-viscosity family of src/incompressible/sgs_models).
 //
 // From the resolved velocity gradient A_ij = du_i/dx_j (igrid's duidxj

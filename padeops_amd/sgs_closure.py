@@ -21,7 +21,7 @@ filter width Delta, plus Delta itself:
 with U the rms of Delta |grad bar(u)| over the field and Delta_0 the finest
 filter width of the dataset.
 
-Judge: the Smagorinsky stress -2 (C Delta)^2 |S| S_ij with its coefficient
+Baseline: the Smagorinsky stress -2 (C Delta)^2 |S| S_ij with its coefficient
 fitted by least squares to the exact stress (the best any constant-coefficient
 eddy viscosity, the dynamic procedure's planar constant included, can do a
 priori in homogeneous turbulence), compared with the network on held-out

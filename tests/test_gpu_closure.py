@@ -26,7 +26,6 @@ import padeops_amd as P
 
 pytestmark = pytest.mark.gpu
 
-OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
 
 
 def _data(seed, cuda):
@@ -39,7 +38,7 @@ def _data(seed, cuda):
     return [C.filtered_dataset(u, v, w, n, f * dx, d0=4 * dx) for f in (4, 6)]
 
 
-def test_closure_beats_smagorinsky_a_priori(cuda):
+def test_closure_beats_smagorinsky_a_priori(cuda, tmp_path):
     import torch
     from padeops_amd import sgs_closure as C
     train = _data(11, cuda)
@@ -60,8 +59,7 @@ def test_closure_beats_smagorinsky_a_priori(cuda):
                                   "smag_ls_coefficient": coef, "gradient_model_corr": cg,
                                   "gradient_model_rel_err": eg}
         print(kc, res["delta_%ddx" % kc], flush=True)
-    os.makedirs(OUT, exist_ok=True)
-    with open(os.path.join(OUT, "closure_apriori.json"), "w") as fh:
+    with open(os.path.join(str(tmp_path), "closure_apriori.json"), "w") as fh:
         json.dump(res, fh, indent=1)
     for kc in (4, 6):
         r = res["delta_%ddx" % kc]
